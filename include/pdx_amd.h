@@ -1,0 +1,151 @@
+/*
+ * pdx_amd.h -- C-ABI of the MI355X (gfx950) PDX search path.
+ *
+ * This is the drop-in boundary under the reference's C++ index API (include/pdx/index.hpp):
+ * the host-side classes (IPDXIndex, PDXIndex<Q>, PDXTreeIndex<Q>) keep their interface and
+ * hand their clusters to this library, which keeps them resident in HBM and runs the search
+ * (probe, ADSampling-pruned PDX scan, top-k) as hand-written HIP kernels.
+ *
+ * Conventions: every call returns a pdxg_status (0 = success) and never throws; the message of
+ * the last failure on the calling thread is available from pdxg_last_error(). Host buffers are
+ * only read during the call. Device memory belongs to the index / workspace handles.
+ */
+#ifndef PDX_AMD_H
+#define PDX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PDXG_OK = 0,
+    PDXG_ERR_INVALID = 1,     /* bad argument or inconsistent index description */
+    PDXG_ERR_HIP = 2,         /* a HIP runtime call failed */
+    PDXG_ERR_UNSUPPORTED = 3, /* valid input outside what the GPU path implements */
+    PDXG_ERR_NOMEM = 4        /* device allocation failed */
+} pdxg_status;
+
+/* PDXIndexType (include/pdx/common.hpp:78). */
+enum { PDXG_F32 = 0, PDXG_U8 = 1, PDXG_TREE_F32 = 2, PDXG_TREE_U8 = 3 };
+
+typedef struct pdxg_index pdxg_index;
+typedef struct pdxg_workspace pdxg_workspace;
+
+/* One cluster as the reference holds it in memory (Cluster<Q>, include/pdx/cluster.hpp:17-105):
+ * `data` is the max_capacity-strided PDX buffer (layout.hpp:21-87, cluster.hpp:341-384),
+ * `ids` the row id of each used slot, `tombstones` the deleted slot indices. */
+typedef struct {
+    uint32_t num_embeddings; /* live vectors (Cluster::num_embeddings) */
+    uint32_t used_capacity;  /* slots in use incl. tombstones (Cluster::used_capacity) */
+    uint32_t max_capacity;   /* stride of the PDX buffer (Cluster::max_capacity) */
+    const void* data;        /* max_capacity * d elements (float for F32 levels, uint8 for U8) */
+    const uint32_t* ids;     /* used_capacity row ids */
+    const uint32_t* tombstones;
+    uint32_t num_tombstones;
+} pdxg_cluster_desc;
+
+/* One IVF level (IVF<Q>, include/pdx/ivf_wrapper.hpp:15-179). */
+typedef struct {
+    uint32_t num_clusters;
+    const pdxg_cluster_desc* clusters;
+    const float* centroids; /* num_clusters * d row-major, or NULL (two-level L1 after restore) */
+} pdxg_level_desc;
+
+/* A whole index: replaces the state PDXIndex<Q>/PDXTreeIndex<Q> keep after Restore/BuildIndex
+ * (index.hpp:241-263, 516-543) plus the ADSampling pruner built from the stored matrix
+ * (pruners/adsampling.hpp:60-78). */
+typedef struct {
+    uint32_t index_type; /* PDXG_F32 .. PDXG_TREE_U8 */
+    uint32_t num_dimensions;
+    uint32_t num_vertical_dimensions;   /* GetPDXDimensionSplit, common.hpp:133-159 */
+    uint32_t num_horizontal_dimensions;
+    uint32_t is_normalized;             /* IVF::is_normalized */
+    float quantization_base;            /* U8 levels (ivf_wrapper.hpp:34-37) */
+    float quantization_scale;
+    const float* rotation; /* d x d row-major (y = x . M) */
+    pdxg_level_desc l1;    /* data clusters */
+    pdxg_level_desc l0;    /* meso clusters over the L1 centroids (tree types only, F32) */
+} pdxg_index_desc;
+
+/* Upload an index (IVF<Q>::Load + ADSamplingPruner ctor). Replaces nothing in the reference
+ * one-to-one; it is the residency step behind PDXIndex::Restore / BuildIndex. */
+pdxg_status pdxg_index_create(int device, const pdxg_index_desc* desc, pdxg_index** out);
+pdxg_status pdxg_index_destroy(pdxg_index* index);
+
+/* Re-upload one cluster after Append/Delete/compaction on the host
+ * (Cluster::AppendEmbedding / DeleteEmbedding / CompactCluster, cluster.hpp:121-154,260-294).
+ * level: 1 = data clusters, 0 = meso clusters. max_capacity must be unchanged. */
+pdxg_status pdxg_index_update_cluster(pdxg_index* index, uint32_t level, uint32_t cluster_id,
+                                      const pdxg_cluster_desc* cluster);
+
+/* Device bytes held by the index. */
+size_t pdxg_index_device_bytes(const pdxg_index* index);
+
+/* Search arguments. For pdxg_search every pointer is a HOST pointer; for pdxg_search_device every
+ * pointer is a DEVICE pointer. k / nprobe may be NULL, then k_all / nprobe_all apply to every query.
+ * nprobe == 0 means "all clusters" (searcher.hpp:614-616, index.hpp:547-550). */
+typedef struct {
+    uint32_t num_queries;
+    const float* queries;          /* num_queries * d */
+    uint32_t queries_transformed;  /* 1: already normalised + rotated (searcher.hpp:596-605) */
+    const uint32_t* k;
+    const uint32_t* nprobe;
+    uint32_t k_all;
+    uint32_t nprobe_all;
+    uint32_t k_max;                /* upper bound of every k (sizes on-chip top-k buffers) */
+    uint32_t nprobe_max;           /* upper bound of every nprobe (0 = may be "all") */
+    uint32_t out_stride;           /* row stride of the outputs, >= k_max */
+    uint32_t* out_ids;             /* num_queries * out_stride */
+    float* out_distances;          /* num_queries * out_stride */
+    uint32_t* out_int_distances;   /* optional (U8 levels): exact integer distances */
+    uint32_t* out_counts;          /* num_queries: results per query (<= k) */
+    uint64_t* out_bytes;           /* optional: algorithmic bytes read per query */
+} pdxg_search_args;
+
+/* Batched search with host buffers; synchronous. Replaces PDXearch::Search driven by
+ * PDXIndex::Search (index.hpp:269-271) and PDXTreeIndex::Search (index.hpp:545-566), one row per
+ * query. Not reentrant on one index handle (nor is the reference's Search). */
+pdxg_status pdxg_search(pdxg_index* index, const pdxg_search_args* args);
+
+/* Scratch for pdxg_search_device, sized for at most max_queries queries with k <= max_k and
+ * nprobe <= max_nprobe (0 = all clusters). */
+pdxg_status pdxg_workspace_create(pdxg_index* index, uint32_t max_queries, uint32_t max_k,
+                                  uint32_t max_nprobe, pdxg_workspace** out);
+pdxg_status pdxg_workspace_destroy(pdxg_workspace* ws);
+/* Device bytes a workspace of that size takes (for admission control). */
+size_t pdxg_workspace_bytes(const pdxg_index* index, uint32_t max_queries, uint32_t max_k,
+                            uint32_t max_nprobe);
+
+/* Asynchronous batched search on `stream` (a hipStream_t, NULL = default stream) with device
+ * buffers. The workspace must not be shared by calls that overlap in time. */
+pdxg_status pdxg_search_device(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,
+                               void* stream);
+
+/* Query preprocessing only (normalise if the index is normalised, rotate; searcher.hpp:601-613),
+ * host buffers: out_rotated num_queries*d floats, out_quantized (U8 indexes, may be NULL)
+ * num_queries*d bytes (ScalarQuantizer::QuantizeEmbedding, quantizers/scalar.hpp:76-94). */
+pdxg_status pdxg_preprocess(pdxg_index* index, const float* queries, uint32_t num_queries,
+                            uint32_t queries_transformed, float* out_rotated, uint8_t* out_quantized);
+
+/* Filtered search of one query (PDXearch::FilteredSearch, searcher.hpp:726-856, as driven by
+ * PDXIndex::FilteredSearch, index.hpp:273-280). selection: one byte per slot of the data level
+ * in cluster-offset order (PredicateEvaluator::selection_vector, db_mock/predicate_evaluator.hpp;
+ * offsets = running sum of max_capacity, ivf_wrapper.hpp:76-87), host buffer. */
+pdxg_status pdxg_filtered_search(pdxg_index* index, const float* query, uint32_t query_transformed,
+                                 uint32_t k, uint32_t nprobe, const uint8_t* selection,
+                                 uint32_t* out_ids, float* out_distances, uint32_t* out_int_distances,
+                                 uint32_t* out_count);
+
+const char* pdxg_last_error(void);
+
+/* Library / device description: "gfx950 <name> <CUs> CUs <LDS> B LDS". */
+pdxg_status pdxg_device_info(int device, char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDX_AMD_H */

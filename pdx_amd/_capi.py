@@ -1,0 +1,261 @@
+"""ctypes binding of the C-ABI in include/pdx_amd.h (libpdx_amd.so).
+
+The library is the only compute path: if it is missing or no GPU is visible, every call raises.
+torch is imported first when available so that the library binds to the same HIP runtime
+(libamdhip64.so.7) as torch, which lets device tensors be handed over by pointer.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+try:  # one HIP runtime per process: let torch load it first
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+from . import pdxfile as pf
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdx_amd.so")
+
+PDXG_OK = 0
+_STATUS = {1: "invalid argument", 2: "HIP error", 3: "unsupported on the GPU path", 4: "out of device memory"}
+
+
+class PdxgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class ClusterDesc(ctypes.Structure):
+    _fields_ = [("num_embeddings", ctypes.c_uint32), ("used_capacity", ctypes.c_uint32),
+                ("max_capacity", ctypes.c_uint32), ("data", ctypes.c_void_p),
+                ("ids", ctypes.POINTER(ctypes.c_uint32)), ("tombstones", ctypes.POINTER(ctypes.c_uint32)),
+                ("num_tombstones", ctypes.c_uint32)]
+
+
+class LevelDesc(ctypes.Structure):
+    _fields_ = [("num_clusters", ctypes.c_uint32), ("clusters", ctypes.POINTER(ClusterDesc)),
+                ("centroids", ctypes.POINTER(ctypes.c_float))]
+
+
+class IndexDesc(ctypes.Structure):
+    _fields_ = [("index_type", ctypes.c_uint32), ("num_dimensions", ctypes.c_uint32),
+                ("num_vertical_dimensions", ctypes.c_uint32), ("num_horizontal_dimensions", ctypes.c_uint32),
+                ("is_normalized", ctypes.c_uint32), ("quantization_base", ctypes.c_float),
+                ("quantization_scale", ctypes.c_float), ("rotation", ctypes.POINTER(ctypes.c_float)),
+                ("l1", LevelDesc), ("l0", LevelDesc)]
+
+
+class SearchArgs(ctypes.Structure):
+    _fields_ = [("num_queries", ctypes.c_uint32), ("queries", ctypes.c_void_p),
+                ("queries_transformed", ctypes.c_uint32), ("k", ctypes.c_void_p), ("nprobe", ctypes.c_void_p),
+                ("k_all", ctypes.c_uint32), ("nprobe_all", ctypes.c_uint32), ("k_max", ctypes.c_uint32),
+                ("nprobe_max", ctypes.c_uint32), ("out_stride", ctypes.c_uint32), ("out_ids", ctypes.c_void_p),
+                ("out_distances", ctypes.c_void_p), ("out_int_distances", ctypes.c_void_p),
+                ("out_counts", ctypes.c_void_p), ("out_bytes", ctypes.c_void_p)]
+
+
+# every symbol include/pdx_amd.h declares
+EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_index_device_bytes",
+           "pdxg_search", "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_workspace_bytes",
+           "pdxg_search_device", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
+           "pdxg_device_info"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built (python -m pdx_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, f32p, u8p = ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint8)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.pdxg_last_error.restype = ctypes.c_char_p
+    L.pdxg_index_create.argtypes = [ctypes.c_int, ctypes.POINTER(IndexDesc), ctypes.POINTER(vp)]
+    L.pdxg_index_destroy.argtypes = [vp]
+    L.pdxg_index_update_cluster.argtypes = [vp, u32, u32, ctypes.POINTER(ClusterDesc)]
+    L.pdxg_index_device_bytes.restype = ctypes.c_size_t
+    L.pdxg_index_device_bytes.argtypes = [vp]
+    L.pdxg_search.argtypes = [vp, ctypes.POINTER(SearchArgs)]
+    L.pdxg_workspace_create.argtypes = [vp, u32, u32, u32, ctypes.POINTER(vp)]
+    L.pdxg_workspace_destroy.argtypes = [vp]
+    L.pdxg_workspace_bytes.restype = ctypes.c_size_t
+    L.pdxg_workspace_bytes.argtypes = [vp, u32, u32, u32]
+    L.pdxg_search_device.argtypes = [vp, vp, ctypes.POINTER(SearchArgs), vp]
+    L.pdxg_preprocess.argtypes = [vp, f32p, u32, u32, f32p, u8p]
+    L.pdxg_filtered_search.argtypes = [vp, f32p, u32, u32, u32, u8p, u32p, f32p, u32p, u32p]
+    L.pdxg_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    for name in ("pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_search",
+                 "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device", "pdxg_preprocess",
+                 "pdxg_filtered_search", "pdxg_device_info"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != PDXG_OK:
+        raise PdxgError(status, lib().pdxg_last_error().decode())
+
+
+def device_info(device: int = 0) -> str:
+    buf = ctypes.create_string_buffer(256)
+    check(lib().pdxg_device_info(device, buf, 256))
+    return buf.value.decode()
+
+
+def _ptr(a: np.ndarray, t=ctypes.c_void_p):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _cluster_desc(c: pf.Cluster, keep: list) -> ClusterDesc:
+    tomb = np.array(sorted(c.tombstones), np.uint32)
+    data = np.ascontiguousarray(c.data)
+    ids = np.ascontiguousarray(c.ids, np.uint32)
+    keep += [tomb, data, ids]
+    return ClusterDesc(c.n_live, c.n_used, c.cap, data.ctypes.data_as(ctypes.c_void_p),
+                       ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                       tomb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(tomb))
+
+
+def _level_desc(L: pf.Level | None, keep: list) -> LevelDesc:
+    if L is None:
+        return LevelDesc(0, None, None)
+    arr = (ClusterDesc * L.nc)(*[_cluster_desc(c, keep) for c in L.clusters])
+    keep.append(arr)
+    cent = None
+    if L.centroids is not None:
+        ca = np.ascontiguousarray(L.centroids, np.float32)
+        keep.append(ca)
+        cent = ca.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    return LevelDesc(L.nc, arr, cent)
+
+
+class GpuIndex:
+    """A device-resident PDX index (pdxg_index) built from an in-memory IndexData."""
+
+    def __init__(self, index: pf.IndexData, device: int = 0):
+        self.data = index
+        self.device = device
+        keep = []
+        m = np.ascontiguousarray(index.matrix, np.float32)
+        keep.append(m)
+        L = index.l1
+        desc = IndexDesc(index.type, L.d, L.v, L.h, int(L.normalized), L.base, L.scale,
+                         m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _level_desc(L, keep),
+                         _level_desc(index.l0, keep))
+        h = ctypes.c_void_p()
+        check(lib().pdxg_index_create(device, ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pdxg_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().pdxg_index_device_bytes(self._h))
+
+    def update_cluster(self, level: int, cluster_id: int) -> None:
+        L = self.data.l1 if level == 1 else self.data.l0
+        keep = []
+        cd = _cluster_desc(L.clusters[cluster_id], keep)
+        check(lib().pdxg_index_update_cluster(self._h, level, cluster_id, ctypes.byref(cd)))
+
+    def search(self, queries: np.ndarray, k, nprobe, transformed: bool = False, want_bytes: bool = False,
+               want_int: bool = True):
+        """Batched search; returns (ids [nq,kmax], dists, int_dists, counts, bytes)."""
+        d = self.data.d
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, d)
+        nq = q.shape[0]
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
+        npb = np.ascontiguousarray(np.broadcast_to(np.asarray(nprobe, np.uint32), (nq,)))
+        kmax = max(1, int(kk.max()) if nq else 1)
+        npmax = 0 if (nq == 0 or (npb == 0).any()) else int(npb.max())
+        ids = np.zeros((nq, kmax), np.uint32)
+        dists = np.zeros((nq, kmax), np.float32)
+        idists = np.zeros((nq, kmax), np.uint32) if want_int else None
+        counts = np.zeros(nq, np.uint32)
+        nbytes = np.zeros(nq, np.uint64) if want_bytes else None
+        a = SearchArgs(nq, _ptr(q), int(transformed), _ptr(kk), _ptr(npb), 0, 0, kmax, npmax, kmax,
+                       _ptr(ids), _ptr(dists), _ptr(idists), _ptr(counts), _ptr(nbytes))
+        check(lib().pdxg_search(self._h, ctypes.byref(a)))
+        return ids, dists, idists, counts, nbytes
+
+    def preprocess(self, queries: np.ndarray, transformed: bool = False):
+        d = self.data.d
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, d)
+        rot = np.zeros_like(q)
+        qu = np.zeros(q.shape, np.uint8) if self.data.is_u8 else None
+        check(lib().pdxg_preprocess(self._h, _ptr(q, ctypes.POINTER(ctypes.c_float)), q.shape[0], int(transformed),
+                                    _ptr(rot, ctypes.POINTER(ctypes.c_float)),
+                                    _ptr(qu, ctypes.POINTER(ctypes.c_uint8)) if qu is not None else None))
+        return rot, qu
+
+    def filtered_search(self, query: np.ndarray, k: int, nprobe: int, selection: np.ndarray,
+                        transformed: bool = False):
+        q = np.ascontiguousarray(query, np.float32).reshape(self.data.d)
+        sel = np.ascontiguousarray(selection, np.uint8)
+        ids = np.zeros(k, np.uint32)
+        dists = np.zeros(k, np.float32)
+        idists = np.zeros(k, np.uint32)
+        cnt = np.zeros(1, np.uint32)
+        check(lib().pdxg_filtered_search(self._h, _ptr(q, ctypes.POINTER(ctypes.c_float)), int(transformed), k,
+                                         nprobe, _ptr(sel, ctypes.POINTER(ctypes.c_uint8)),
+                                         _ptr(ids, ctypes.POINTER(ctypes.c_uint32)),
+                                         _ptr(dists, ctypes.POINTER(ctypes.c_float)),
+                                         _ptr(idists, ctypes.POINTER(ctypes.c_uint32)),
+                                         _ptr(cnt, ctypes.POINTER(ctypes.c_uint32))))
+        n = int(cnt[0])
+        return ids[:n], dists[:n], idists[:n]
+
+
+class Workspace:
+    """pdxg_workspace for device-buffer searches (pdxg_search_device)."""
+
+    def __init__(self, index: GpuIndex, max_queries: int, max_k: int, max_nprobe: int):
+        self.index = index
+        h = ctypes.c_void_p()
+        check(lib().pdxg_workspace_create(index.handle, max_queries, max_k, max_nprobe, ctypes.byref(h)))
+        self._h = h
+        self.bytes = int(lib().pdxg_workspace_bytes(index.handle, max_queries, max_k, max_nprobe))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pdxg_workspace_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def search(self, queries_ptr: int, nq: int, k_ptr: int, nprobe_ptr: int, k_max: int, nprobe_max: int,
+               out_stride: int, out_ids: int, out_dists: int, out_idists: int, out_counts: int,
+               out_bytes: int = 0, transformed: bool = False, stream: int = 0, k_all: int = 0,
+               nprobe_all: int = 0) -> None:
+        """Asynchronous search on device pointers (ints; 0 = NULL) on `stream` (hipStream_t)."""
+        a = SearchArgs(nq, queries_ptr or None, int(transformed), k_ptr or None, nprobe_ptr or None, k_all,
+                       nprobe_all, k_max, nprobe_max, out_stride, out_ids or None, out_dists or None,
+                       out_idists or None, out_counts or None, out_bytes or None)
+        check(lib().pdxg_search_device(self.index.handle, self._h, ctypes.byref(a), ctypes.c_void_p(stream or 0)))

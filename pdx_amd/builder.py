@@ -1,0 +1,178 @@
+"""Index construction (the BuildIndex pipeline of include/pdx/index.hpp:329-394 and 640-757).
+
+normalise (if required) -> random orthogonal rotation (FP64 QR of a seeded Gaussian matrix,
+pruners/adsampling.hpp:44-57) -> global SQ8 parameters (quantizers/scalar.hpp:60-74) -> k-means
+-> PDX clusters (index.hpp:101-163) -> for two-level indexes, k-means over the centroids with
+floor(sqrt(nlist)) meso clusters whose members are the L1 centroids (index.hpp:713-757).
+
+The reference clusters with SuperKMeans (absent from the reference tree); this module runs a
+seeded Lloyd k-means of its own (sampled training set, spherical when the data is normalised).
+Search parity does not depend on the clustering: any valid index in the reference layout is
+searched identically by the oracle and the GPU path. Heavy products run through torch on the
+GPU when one is present, numpy otherwise.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import pdxfile as pf
+
+MIN_EMBEDDINGS_TO_SAMPLE = 30720  # clustering.hpp:22
+METRICS = {"l2sq": 0, "cosine": 1, "ip": 2}
+
+
+def compute_number_of_clusters(n: int) -> int:
+    """common.hpp:191-199."""
+    if n < 500000:
+        return int(math.ceil(2 * math.sqrt(n)))
+    if n < 2500000:
+        return int(math.ceil(4 * math.sqrt(n)))
+    return int(math.ceil(8 * math.sqrt(n)))
+
+
+def random_rotation(d: int, seed: int) -> np.ndarray:
+    """Random orthogonal matrix, QR in float64 with the sign of diag(R) folded into Q."""
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((d, d))
+    q, r = np.linalg.qr(g)
+    q *= np.sign(np.diag(r))[None, :]
+    return np.ascontiguousarray(q.astype(np.float32))
+
+
+def normalize_rows(x: np.ndarray) -> np.ndarray:
+    n = np.sqrt(np.einsum("ij,ij->i", x, x, dtype=np.float32)).astype(np.float32)
+    n[n == 0] = 1.0
+    return (x / n[:, None]).astype(np.float32)
+
+
+def quantization_params(y: np.ndarray):
+    """Global (base, scale): ScalarQuantizer::ComputeQuantizationParams (scalar.hpp:60-74)."""
+    lo = np.float32(y.min())
+    hi = np.float32(y.max())
+    rng = np.float32(hi - lo)
+    scale = np.float32(255.0) / rng if rng > 0 else np.float32(1.0)
+    return float(lo), float(scale)
+
+
+def quantize(y: np.ndarray, base: float, scale: float) -> np.ndarray:
+    """clamp(round_half_away((y - base) * scale), 0, 255) in float32 (scalar.hpp:76-94)."""
+    t = (y.astype(np.float32) - np.float32(base)) * np.float32(scale)
+    r = np.sign(t) * np.floor(np.abs(t) + np.float32(0.5))
+    return np.clip(r, 0, 255).astype(np.uint8)
+
+
+class _Backend:
+    def __init__(self, device=None):
+        self.torch = None
+        self.dev = None
+        try:
+            import torch
+
+            if device is not False and torch.cuda.is_available():
+                self.torch = torch
+                self.dev = torch.device("cuda", 0 if device in (None, True) else int(device))
+        except ImportError:
+            pass
+
+    def matmul(self, a: np.ndarray, b: np.ndarray, chunk: int = 1 << 18) -> np.ndarray:
+        if self.torch is None:
+            return a @ b
+        t = self.torch
+        bt = t.from_numpy(np.ascontiguousarray(b)).to(self.dev)
+        out = np.empty((a.shape[0], b.shape[1]), np.float32)
+        for s in range(0, a.shape[0], chunk):
+            at = t.from_numpy(np.ascontiguousarray(a[s:s + chunk])).to(self.dev)
+            out[s:s + chunk] = (at @ bt).cpu().numpy()
+        return out
+
+    def assign(self, x: np.ndarray, c: np.ndarray, chunk: int = 1 << 16) -> np.ndarray:
+        """Nearest centroid by L2 (ties: lowest index)."""
+        cn = np.einsum("ij,ij->i", c, c).astype(np.float32)
+        out = np.empty(x.shape[0], np.int64)
+        if self.torch is None:
+            for s in range(0, x.shape[0], chunk):
+                xs = x[s:s + chunk]
+                dd = cn[None, :] - 2.0 * (xs @ c.T)
+                out[s:s + chunk] = np.argmin(dd, axis=1)
+            return out
+        t = self.torch
+        ct = t.from_numpy(np.ascontiguousarray(c)).to(self.dev)
+        cnt = t.from_numpy(cn).to(self.dev)
+        for s in range(0, x.shape[0], chunk):
+            xt = t.from_numpy(np.ascontiguousarray(x[s:s + chunk])).to(self.dev)
+            dd = cnt[None, :] - 2.0 * (xt @ ct.T)
+            out[s:s + chunk] = t.argmin(dd, dim=1).cpu().numpy()
+        return out
+
+
+def kmeans(x: np.ndarray, k: int, iters: int, seed: int, sampling_fraction: float, angular: bool,
+           backend: _Backend):
+    """Seeded Lloyd k-means; returns (centroids [k,d] float32, assignment of every row)."""
+    n, d = x.shape
+    rng = np.random.default_rng(seed)
+    if k == 1:
+        return x[:1].copy(), np.zeros(n, np.int64)
+    frac = sampling_fraction if sampling_fraction > 0 else (1.0 if n < MIN_EMBEDDINGS_TO_SAMPLE else 0.3)
+    ns = max(k, min(n, int(n * frac)))
+    train = x[rng.choice(n, ns, replace=False)] if ns < n else x
+    cent = train[rng.choice(train.shape[0], k, replace=False)].astype(np.float32).copy()
+    for _ in range(iters):
+        a = backend.assign(train, cent)
+        counts = np.bincount(a, minlength=k).astype(np.float32)
+        sums = np.zeros((k, d), np.float64)
+        np.add.at(sums, a, train)
+        empty = counts == 0
+        cent = (sums / np.maximum(counts, 1)[:, None]).astype(np.float32)
+        if empty.any():
+            cent[empty] = train[rng.choice(train.shape[0], int(empty.sum()), replace=False)]
+        if angular:
+            cent = normalize_rows(cent)
+    return cent, backend.assign(x, cent)
+
+
+def _clusters_from_assignment(vecs, assign, k, level, row_ids):
+    order = np.argsort(assign, kind="stable")
+    bounds = np.searchsorted(assign[order], np.arange(k + 1))
+    clusters = []
+    for c in range(k):
+        members = order[bounds[c]:bounds[c + 1]]
+        clusters.append(pf.make_cluster(vecs[members], row_ids[members].astype(np.uint32), level))
+    return clusters
+
+
+def build_index(data: np.ndarray, index_type: str = "pdx_tree_u8", *, distance_metric: str = "l2sq",
+                normalize: bool = False, seed: int = 42, num_clusters: int = 0, num_meso_clusters: int = 0,
+                sampling_fraction: float = 0.0, kmeans_iters: int = 10, device=None,
+                rotation: np.ndarray | None = None) -> pf.IndexData:
+    """Build an index of `index_type` over `data` (n x d float32) with row ids 0..n-1."""
+    t = pf.TYPE_NAMES[index_type]
+    x = np.ascontiguousarray(data, np.float32)
+    n, d = x.shape
+    if d == 0 or d > 16384:
+        raise ValueError("num_dimensions must be between 1 and 16384")
+    if num_clusters and num_clusters > n:
+        raise ValueError(f"num_clusters ({num_clusters}) exceeds num_embeddings ({n})")
+    nlist = num_clusters or compute_number_of_clusters(n)
+    norm = bool(normalize) or distance_metric in ("cosine", "ip")
+    be = _Backend(device)
+    m = random_rotation(d, seed) if rotation is None else np.ascontiguousarray(rotation, np.float32)
+    y = be.matmul(normalize_rows(x) if norm else x, m)
+    h, v = pf.dimension_split(d)
+    u8 = t in (pf.TYPE_U8, pf.TYPE_TREE_U8)
+    base, scale = quantization_params(y) if u8 else (0.0, 1.0)
+    cent, assign = kmeans(y, nlist, kmeans_iters, seed, sampling_fraction, norm, be)
+    L = pf.Level(d, v, h, u8, norm, [], cent, base, scale)
+    stored = quantize(y, base, scale) if u8 else y
+    L.clusters = _clusters_from_assignment(stored, assign, nlist, L, np.arange(n))
+    idx = pf.IndexData(type=t, matrix=m, l1=L)
+    if t >= 2:
+        n0 = num_meso_clusters or int(math.sqrt(nlist))
+        if n0 >= nlist:
+            raise ValueError("num_meso_clusters must be smaller than num_clusters")
+        c0, a0 = kmeans(cent, n0, 10, seed, 1.0, norm, be)
+        M = pf.Level(d, v, h, False, norm, [], c0)
+        M.clusters = _clusters_from_assignment(cent, a0, n0, M, np.arange(nlist))
+        idx.l0 = M
+    return idx

@@ -1,0 +1,806 @@
+// C-ABI of the MI355X PDX search path (include/pdx_amd.h): device residency of an index,
+// per-call workspaces and the launch sequence of the search kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pdx_amd.h"
+#include "pdx_kernels.hpp"
+
+namespace {
+
+using namespace pdxg;
+
+thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+    pdxg_status code;
+    Error(pdxg_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw Error(e_ == hipErrorOutOfMemory ? PDXG_ERR_NOMEM : PDXG_ERR_HIP,               \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+    } while (0)
+
+void require(bool ok, const std::string& msg, pdxg_status code = PDXG_ERR_INVALID) {
+    if (!ok) throw Error(code, msg);
+}
+
+template <class F>
+pdxg_status guarded(F&& f) {
+    try {
+        f();
+        return PDXG_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PDXG_ERR_INVALID;
+    }
+}
+
+// On-chip top-k buffers are sized by the largest k of a call; beyond this the GPU path reports
+// PDXG_ERR_UNSUPPORTED rather than silently degrading.
+constexpr uint32_t kMaxK = 2048;
+constexpr uint32_t kScanBlock = 256;
+constexpr uint32_t kMaxProbeSort = 8192;  // k_probe_select sorts next_pow2(nc) keys in LDS
+
+// adsampling.hpp:147-157
+float ratio_at(size_t i, uint32_t D) {
+    if (i == 0 || i == D) return 1.0f;
+    double f = static_cast<double>(static_cast<float>(i) / static_cast<float>(D));
+    double a = 1.0 + 1.5 / std::sqrt(static_cast<double>(i));
+    return static_cast<float>(f * a * a);
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) {
+        if (count > n || !p) alloc(std::max(count, n * 5 / 4));
+    }
+    void upload(const T* h, size_t count, size_t off = 0) {
+        if (count) HIP_CHECK(hipMemcpy(p + off, h, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Level {
+    uint32_t nc = 0, d = 0, v = 0, h = 0;
+    bool u8 = false;
+    bool has_centroids = false;
+    std::vector<uint32_t> n_used, n_live, cap, n_tomb;
+    std::vector<uint64_t> data_off, slot_off, tomb_off;
+    uint64_t data_bytes = 0, total_slots = 0, tomb_words = 0;
+    DevBuf<uint8_t> data;
+    DevBuf<uint64_t> d_data_off, d_slot_off, d_tomb_off;
+    DevBuf<uint32_t> ids, d_n_used, d_n_live, d_cap, tomb, d_n_tomb;
+    DevBuf<float> centroids;
+
+    size_t elem() const { return u8 ? 1 : 4; }
+    uint32_t max_used() const {
+        uint32_t m = 1;
+        for (uint32_t x : n_used) m = std::max(m, x);
+        return m;
+    }
+    size_t device_bytes() const {
+        return data.bytes() + d_data_off.bytes() + d_slot_off.bytes() + d_tomb_off.bytes() + ids.bytes() +
+               d_n_used.bytes() + d_n_live.bytes() + d_cap.bytes() + tomb.bytes() + d_n_tomb.bytes() +
+               centroids.bytes();
+    }
+
+    void check_cluster(const pdxg_cluster_desc& c) const {
+        require(c.max_capacity > 0, "cluster max_capacity must be positive");
+        require(c.used_capacity <= c.max_capacity, "used_capacity exceeds max_capacity");
+        require(c.num_embeddings <= c.used_capacity, "num_embeddings exceeds used_capacity");
+        require(c.num_embeddings + c.num_tombstones == c.used_capacity,
+                "num_embeddings + tombstones must equal used_capacity");
+        require(c.used_capacity == 0 || (c.data && c.ids), "cluster data/ids missing");
+        require((uint64_t)c.max_capacity * d < (1ull << 32), "cluster buffer too large");
+        for (uint32_t t = 0; t < c.num_tombstones; ++t)
+            require(c.tombstones[t] < c.used_capacity, "tombstone slot out of range");
+    }
+
+    void upload_cluster(uint32_t c, const pdxg_cluster_desc& cd) {
+        const size_t bytes = (size_t)cd.max_capacity * d * elem();
+        if (cd.used_capacity) {
+            HIP_CHECK(hipMemcpy(data.p + data_off[c], cd.data, bytes, hipMemcpyHostToDevice));
+            ids.upload(cd.ids, cd.used_capacity, slot_off[c]);
+        }
+        const uint32_t words = (cd.max_capacity + 31) / 32;
+        std::vector<uint32_t> bm(words, 0u);
+        for (uint32_t t = 0; t < cd.num_tombstones; ++t) bm[cd.tombstones[t] >> 5] |= 1u << (cd.tombstones[t] & 31);
+        tomb.upload(bm.data(), words, tomb_off[c]);
+        n_used[c] = cd.used_capacity;
+        n_live[c] = cd.num_embeddings;
+        n_tomb[c] = cd.num_tombstones;
+    }
+
+    void create(const pdxg_level_desc& L, uint32_t dd, uint32_t vv, uint32_t hh, bool is_u8) {
+        nc = L.num_clusters;
+        d = dd;
+        v = vv;
+        h = hh;
+        u8 = is_u8;
+        require(nc > 0, "level has no clusters");
+        require(L.clusters != nullptr, "level clusters missing");
+        n_used.resize(nc);
+        n_live.resize(nc);
+        cap.resize(nc);
+        n_tomb.resize(nc);
+        data_off.resize(nc);
+        slot_off.resize(nc);
+        tomb_off.resize(nc);
+        data_bytes = total_slots = tomb_words = 0;
+        for (uint32_t c = 0; c < nc; ++c) {
+            const auto& cd = L.clusters[c];
+            check_cluster(cd);
+            cap[c] = cd.max_capacity;
+            data_off[c] = data_bytes;
+            data_bytes += ((size_t)cd.max_capacity * d * elem() + 255) / 256 * 256;
+            slot_off[c] = total_slots;
+            total_slots += cd.max_capacity;
+            tomb_off[c] = tomb_words;
+            tomb_words += (cd.max_capacity + 31) / 32;
+        }
+        data.alloc(data_bytes);
+        ids.alloc(total_slots);
+        tomb.alloc(tomb_words);
+        for (uint32_t c = 0; c < nc; ++c) upload_cluster(c, L.clusters[c]);
+        d_data_off.alloc(nc);
+        d_data_off.upload(data_off.data(), nc);
+        d_slot_off.alloc(nc);
+        d_slot_off.upload(slot_off.data(), nc);
+        d_tomb_off.alloc(nc);
+        d_tomb_off.upload(tomb_off.data(), nc);
+        d_cap.alloc(nc);
+        d_cap.upload(cap.data(), nc);
+        d_n_used.alloc(nc);
+        d_n_live.alloc(nc);
+        d_n_tomb.alloc(nc);
+        sync_counts();
+        has_centroids = L.centroids != nullptr;
+        if (has_centroids) {
+            centroids.alloc((size_t)nc * d);
+            centroids.upload(L.centroids, (size_t)nc * d);
+        }
+    }
+
+    void sync_counts() {
+        d_n_used.upload(n_used.data(), nc);
+        d_n_live.upload(n_live.data(), nc);
+        d_n_tomb.upload(n_tomb.data(), nc);
+    }
+
+    LevelDev view(float scale_sq, float inv_scale_sq) const {
+        LevelDev x;
+        x.data = data.p;
+        x.data_off = d_data_off.p;
+        x.ids = ids.p;
+        x.slot_off = d_slot_off.p;
+        x.n_used = d_n_used.p;
+        x.n_live = d_n_live.p;
+        x.cap = d_cap.p;
+        x.tomb = tomb.p;
+        x.tomb_off = d_tomb_off.p;
+        x.n_tomb = d_n_tomb.p;
+        x.centroids = centroids.p;
+        x.nc = nc;
+        x.d = d;
+        x.v = v;
+        x.h = h;
+        x.scale_sq = scale_sq;
+        x.inv_scale_sq = inv_scale_sq;
+        return x;
+    }
+};
+
+struct LdsPlan {
+    uint32_t bytes = 0;
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red;
+};
+
+LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
+    LdsPlan p{};
+    uint32_t off = 0;
+    auto take = [&](uint32_t bytes) {
+        off = (off + 15) / 16 * 16;
+        uint32_t o = off;
+        off += bytes;
+        return o;
+    };
+    const uint32_t nw = (L.v + 3) / 4 + L.h / 4;
+    p.o_q = take(L.u8 ? nw * 4 : L.d * 4);
+    p.o_qq = take(L.u8 ? nw * 4 : 0);
+    const uint32_t capn = L.max_used();
+    p.o_dist = take(capn * 4);
+    p.o_pos = take(capn * 4);
+    p.o_h = take(kmax * 8);
+    p.o_t = take(kmax * 8);
+    p.o_o = take(kmax * 8);
+    p.o_ck = take(block * 8);
+    p.o_hp = take(kmax * 4);
+    p.o_tp = take(kmax * 4);
+    p.o_op = take(kmax * 4);
+    p.o_cp = take(block * 4);
+    p.o_red = take(64);
+    p.bytes = (off + 15) / 16 * 16;
+    return p;
+}
+
+}  // namespace
+
+struct pdxg_workspace {
+    int device = 0;
+    uint32_t max_queries = 0, max_k = 0, max_nprobe = 0;
+    uint32_t k0_max = 0, order_stride = 0;
+    DevBuf<float> qnorm, qrot, probe;
+    DevBuf<uint8_t> qu8;
+    DevBuf<uint32_t> k1, nv1, k0, nv0, order, ids0, cnt0;
+    size_t bytes() const {
+        return qnorm.bytes() + qrot.bytes() + probe.bytes() + qu8.bytes() + k1.bytes() + nv1.bytes() + k0.bytes() +
+               nv0.bytes() + order.bytes() + ids0.bytes() + cnt0.bytes();
+    }
+};
+
+struct pdxg_index {
+    int device = 0;
+    uint32_t type = 0, d = 0, v = 0, h = 0;
+    bool normalized = false;
+    float qbase = 0.0f, qscale = 1.0f, qscale_sq = 1.0f, inv_qscale_sq = 1.0f;
+    Level l1, l0;
+    DevBuf<float> rotation, ratios;
+    int max_lds = 65536;
+    std::mutex mu;  // serialises the host-buffer API (shared workspace)
+    std::unique_ptr<pdxg_workspace> ws;
+    DevBuf<float> h_queries, h_dists;
+    DevBuf<uint32_t> h_k, h_np, h_ids, h_idists, h_counts;
+    DevBuf<unsigned long long> h_bytes;
+    DevBuf<uint8_t> sel;
+    DevBuf<uint32_t> npass, f_order;
+    DevBuf<float> f_probe;
+
+    bool tree() const { return type == PDXG_TREE_F32 || type == PDXG_TREE_U8; }
+    bool u8() const { return type == PDXG_U8 || type == PDXG_TREE_U8; }
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        HIP_CHECK(hipGetDevice(&prev));
+        if (prev != dev) HIP_CHECK(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+uint32_t k0_capacity(const pdxg_index* ix, uint32_t max_nprobe) {
+    if (!ix->tree()) return 0;
+    return (max_nprobe == 0 || max_nprobe > ix->l1.nc) ? ix->l1.nc : max_nprobe;
+}
+
+void plan_workspace(const pdxg_index* ix, uint32_t nq, uint32_t max_k, uint32_t max_nprobe, pdxg_workspace& ws) {
+    ws.max_queries = nq;
+    ws.max_k = max_k;
+    ws.max_nprobe = max_nprobe;
+    ws.k0_max = k0_capacity(ix, max_nprobe);
+    ws.order_stride = ix->tree() ? ix->l0.nc : ((max_nprobe == 0 || max_nprobe > ix->l1.nc) ? ix->l1.nc : max_nprobe);
+}
+
+size_t workspace_bytes_for(const pdxg_index* ix, uint32_t nq, uint32_t max_k, uint32_t max_nprobe) {
+    pdxg_workspace ws;
+    plan_workspace(ix, nq, max_k, max_nprobe, ws);
+    const size_t d = ix->d;
+    const size_t nc_probe = ix->tree() ? ix->l0.nc : ix->l1.nc;
+    size_t b = 0;
+    b += 2 * (size_t)nq * d * 4;          // qnorm, qrot
+    b += (size_t)nq * nc_probe * 4;        // probe
+    b += ix->u8() ? (size_t)nq * d : 0;    // qu8
+    b += 4 * (size_t)nq * 4;               // k1 nv1 k0 nv0
+    b += (size_t)nq * ws.order_stride * 4; // order
+    b += (size_t)nq * ws.k0_max * 4 + (size_t)nq * 4;
+    return b;
+}
+
+void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
+    const size_t nq = ws.max_queries, d = ix->d;
+    const size_t nc_probe = ix->tree() ? ix->l0.nc : ix->l1.nc;
+    ws.qnorm.alloc(nq * d);
+    ws.qrot.alloc(nq * d);
+    ws.probe.alloc(nq * nc_probe);
+    if (ix->u8()) ws.qu8.alloc(nq * d);
+    ws.k1.alloc(nq);
+    ws.nv1.alloc(nq);
+    ws.k0.alloc(nq);
+    ws.nv0.alloc(nq);
+    ws.order.alloc(nq * ws.order_stride);
+    if (ix->tree()) {
+        ws.ids0.alloc(nq * ws.k0_max);
+        ws.cnt0.alloc(nq);
+    }
+}
+
+template <bool U8, bool FILT>
+void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
+    LdsPlan p = plan_scan_lds(L, kmax, kScanBlock);
+    require(p.bytes <= (uint32_t)ix->max_lds,
+            "cluster capacity / k too large for the on-chip scan buffers (" + std::to_string(p.bytes) + " B LDS)",
+            PDXG_ERR_UNSUPPORTED);
+    a.o_q = p.o_q;
+    a.o_qq = p.o_qq;
+    a.o_dist = p.o_dist;
+    a.o_pos = p.o_pos;
+    a.o_h = p.o_h;
+    a.o_hp = p.o_hp;
+    a.o_t = p.o_t;
+    a.o_tp = p.o_tp;
+    a.o_o = p.o_o;
+    a.o_op = p.o_op;
+    a.o_ck = p.o_ck;
+    a.o_cp = p.o_cp;
+    a.o_red = p.o_red;
+    auto kern = k_scan<U8, FILT, kScanBlock>;
+    if (p.bytes > 65536)
+        HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)p.bytes));
+    if (a.nq == 0) return;
+    hipLaunchKernelGGL(kern, dim3(a.nq), dim3(kScanBlock), p.bytes, st, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_probe(const pdxg_index* ix, const Level& L, const float* qrot, uint32_t nq, const uint32_t* nvisit,
+                  uint32_t* order, uint32_t order_stride, float* probe, hipStream_t st) {
+    require(L.has_centroids, "level has no centroids on device", PDXG_ERR_UNSUPPORTED);
+    require(L.nc <= kMaxProbeSort, "too many centroids for the on-chip probe sort", PDXG_ERR_UNSUPPORTED);
+    if (nq == 0) return;
+    dim3 g1(nq, (L.nc + kProbeCPB - 1) / kProbeCPB);
+    hipLaunchKernelGGL(k_probe_dist, g1, dim3(256), ix->d * 4, st, qrot, L.centroids.p, probe, ix->d, L.nc, nq);
+    HIP_CHECK(hipGetLastError());
+    uint32_t n2 = 1;
+    while (n2 < L.nc) n2 <<= 1;
+    if (n2 * 8 > 65536)
+        HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_probe_select),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(n2 * 8)));
+    hipLaunchKernelGGL(k_probe_select, dim3(nq), dim3(256), n2 * 8, st, probe, L.nc, n2, nvisit, order, order_stride,
+                       nq);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_preprocess(const pdxg_index* ix, pdxg_workspace& ws, const float* queries, uint32_t nq,
+                       bool transformed, hipStream_t st) {
+    if (nq == 0) return;
+    const float* src = queries;
+    if (!transformed && ix->normalized) {
+        hipLaunchKernelGGL(k_normalize, dim3(nq), dim3(256), 0, st, queries, ws.qnorm.p, ix->d, nq);
+        HIP_CHECK(hipGetLastError());
+        src = ws.qnorm.p;
+    }
+    dim3 g((ix->d + kRotTI - 1) / kRotTI, (nq + kRotTQ - 1) / kRotTQ);
+    hipLaunchKernelGGL(k_rotate, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p,
+                       ix->u8() ? ws.qu8.p : static_cast<uint8_t*>(nullptr), ix->d, nq, ix->qbase, ix->qscale,
+                       transformed ? 1 : 0);
+    HIP_CHECK(hipGetLastError());
+}
+
+// The full search sequence on device buffers.
+void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a, hipStream_t st) {
+    const uint32_t nq = a.num_queries;
+    require(nq <= ws.max_queries, "more queries than the workspace holds");
+    require(a.k_max >= 1 && a.k_max <= ws.max_k, "k_max must be in [1, workspace max_k]");
+    require(a.k_max <= kMaxK, "k above " + std::to_string(kMaxK) + " is not supported on the GPU path",
+            PDXG_ERR_UNSUPPORTED);
+    require(a.out_stride >= a.k_max, "out_stride must be >= k_max");
+    require(a.out_ids && a.out_counts, "out_ids / out_counts required");
+    const uint32_t np_cap = (a.nprobe_max == 0) ? 0 : a.nprobe_max;
+    require(ws.max_nprobe == 0 || (np_cap != 0 && np_cap <= ws.max_nprobe), "nprobe_max exceeds the workspace");
+    if (!a.k) require(a.k_all >= 1, "k must be >= 1");
+    if (!a.nprobe) require(a.nprobe_all == np_cap || np_cap == 0 || a.nprobe_all <= np_cap, "nprobe_all > nprobe_max");
+    if (nq == 0) return;
+
+    launch_preprocess(ix, ws, a.queries, nq, a.queries_transformed != 0, st);
+
+    SetupArgs sa{};
+    sa.k_in = a.k;
+    sa.nprobe_in = a.nprobe;
+    sa.k_all = a.k_all;
+    sa.nprobe_all = a.nprobe_all;
+    sa.nq = nq;
+    sa.tree = ix->tree();
+    sa.nc1 = ix->l1.nc;
+    sa.nc0 = ix->tree() ? ix->l0.nc : 0;
+    sa.k_cap = a.k_max;
+    sa.order_cap = ws.order_stride;
+    sa.k0_cap = ws.k0_max;
+    sa.k1 = ws.k1.p;
+    sa.nvisit1 = ws.nv1.p;
+    sa.k0 = ws.k0.p;
+    sa.nvisit0 = ws.nv0.p;
+    sa.bytes = reinterpret_cast<unsigned long long*>(a.out_bytes);
+    sa.probe_bytes = (uint64_t)(ix->tree() ? ix->l0.nc : ix->l1.nc) * ix->d * 4;  // centroids streamed per query
+    hipLaunchKernelGGL(k_setup, dim3((nq + 255) / 256), dim3(256), 0, st, sa);
+    HIP_CHECK(hipGetLastError());
+
+    const float* qrot = ws.qrot.p;
+
+    ScanArgs base{};
+    base.ratios = ix->ratios.p;
+    base.out_bytes = reinterpret_cast<unsigned long long*>(a.out_bytes);
+    base.nq = nq;
+
+    if (!ix->tree()) {
+        launch_probe(ix, ix->l1, qrot, nq, ws.nv1.p, ws.order.p, ws.order_stride, ws.probe.p, st);
+        ScanArgs s = base;
+        s.L = ix->l1.view(ix->qscale_sq, ix->inv_qscale_sq);
+        s.qf = qrot;
+        s.qu = ws.qu8.p;
+        s.order = ws.order.p;
+        s.order_stride = ws.order_stride;
+        s.nvisit = ws.nv1.p;
+        s.order_count = nullptr;
+        s.kq = ws.k1.p;
+        s.out_ids = a.out_ids;
+        s.out_dists = a.out_distances;
+        s.out_idists = a.out_int_distances;
+        s.out_counts = a.out_counts;
+        s.out_stride = a.out_stride;
+        if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s, a.k_max, st);
+        else launch_scan<false, false>(ix, ix->l1, s, a.k_max, st);
+    } else {
+        require(ws.k0_max <= kMaxK,
+                "nprobe above " + std::to_string(kMaxK) + " (or 0 with that many clusters) is not supported on the "
+                "GPU path", PDXG_ERR_UNSUPPORTED);
+        launch_probe(ix, ix->l0, qrot, nq, ws.nv0.p, ws.order.p, ws.order_stride, ws.probe.p, st);
+        ScanArgs s0 = base;
+        s0.L = ix->l0.view(1.0f, 1.0f);
+        s0.qf = qrot;
+        s0.order = ws.order.p;
+        s0.order_stride = ws.order_stride;
+        s0.nvisit = ws.nv0.p;
+        s0.kq = ws.k0.p;
+        s0.out_ids = ws.ids0.p;
+        s0.out_dists = nullptr;
+        s0.out_idists = nullptr;
+        s0.out_counts = ws.cnt0.p;
+        s0.out_stride = ws.k0_max;
+        launch_scan<false, false>(ix, ix->l0, s0, ws.k0_max, st);
+        ScanArgs s1 = base;
+        s1.L = ix->l1.view(ix->qscale_sq, ix->inv_qscale_sq);
+        s1.qf = qrot;
+        s1.qu = ws.qu8.p;
+        s1.order = ws.ids0.p;
+        s1.order_stride = ws.k0_max;
+        s1.nvisit = ws.nv1.p;
+        s1.order_count = ws.cnt0.p;
+        s1.kq = ws.k1.p;
+        s1.out_ids = a.out_ids;
+        s1.out_dists = a.out_distances;
+        s1.out_idists = a.out_int_distances;
+        s1.out_counts = a.out_counts;
+        s1.out_stride = a.out_stride;
+        if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s1, a.k_max, st);
+        else launch_scan<false, false>(ix, ix->l1, s1, a.k_max, st);
+    }
+}
+
+pdxg_workspace* host_workspace(pdxg_index* ix, uint32_t nq, uint32_t max_k, uint32_t max_nprobe) {
+    auto& ws = ix->ws;
+    pdxg_workspace want;
+    plan_workspace(ix, nq, max_k, max_nprobe, want);
+    bool ok = ws && ws->max_queries >= nq && ws->max_k >= max_k && ws->k0_max >= want.k0_max &&
+              ws->order_stride >= want.order_stride &&
+              (ws->max_nprobe == 0 || (max_nprobe != 0 && max_nprobe <= ws->max_nprobe));
+    if (!ok) {
+        uint32_t mq = std::max(nq, ws ? ws->max_queries : 0u);
+        uint32_t mk = std::max(max_k, ws ? ws->max_k : 0u);
+        uint32_t mp = (max_nprobe == 0 || (ws && ws->max_nprobe == 0)) ? 0u
+                                                                       : std::max(max_nprobe, ws ? ws->max_nprobe : 0u);
+        ws.reset(new pdxg_workspace);
+        ws->device = ix->device;
+        plan_workspace(ix, mq, mk, mp, *ws);
+        alloc_workspace(ix, *ws);
+    }
+    return ws.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pdxg_last_error(void) { return g_last_error.c_str(); }
+
+pdxg_status pdxg_device_info(int device, char* buf, size_t len) {
+    return guarded([&] {
+        hipDeviceProp_t p;
+        HIP_CHECK(hipGetDeviceProperties(&p, device));
+        int lds = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        if (buf && len) snprintf(buf, len, "%s %s %d CUs %d B LDS", p.gcnArchName, p.name, p.multiProcessorCount, lds);
+    });
+}
+
+pdxg_status pdxg_index_create(int device, const pdxg_index_desc* desc, pdxg_index** out) {
+    return guarded([&] {
+        require(desc && out, "null argument");
+        *out = nullptr;
+        require(desc->index_type <= PDXG_TREE_U8, "unknown index type");
+        require(desc->num_dimensions > 0, "num_dimensions must be positive");
+        require(desc->num_vertical_dimensions + desc->num_horizontal_dimensions == desc->num_dimensions,
+                "vertical + horizontal dimensions must equal num_dimensions");
+        require(desc->num_horizontal_dimensions % kHDim == 0, "horizontal dimensions must be a multiple of 64");
+        require(desc->rotation != nullptr, "rotation matrix required (dense d x d)");
+        DeviceGuard g(device);
+        std::unique_ptr<pdxg_index> ix(new pdxg_index);
+        ix->device = device;
+        ix->type = desc->index_type;
+        ix->d = desc->num_dimensions;
+        ix->v = desc->num_vertical_dimensions;
+        ix->h = desc->num_horizontal_dimensions;
+        ix->normalized = desc->is_normalized != 0;
+        if (ix->u8()) {
+            ix->qbase = desc->quantization_base;
+            ix->qscale = desc->quantization_scale;
+            ix->qscale_sq = ix->qscale * ix->qscale;
+            ix->inv_qscale_sq = 1.0f / ix->qscale_sq;
+        }
+        HIP_CHECK(hipDeviceGetAttribute(&ix->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        ix->l1.create(desc->l1, ix->d, ix->v, ix->h, ix->u8());
+        if (ix->tree()) {
+            // the L0 search returns these ids as the L1 visit order: they must name L1 clusters
+            for (uint32_t c = 0; c < desc->l0.num_clusters; ++c) {
+                const auto& cd = desc->l0.clusters[c];
+                for (uint32_t s = 0; s < cd.used_capacity; ++s)
+                    require(cd.ids[s] < desc->l1.num_clusters, "meso-cluster member id is not an L1 cluster");
+            }
+            ix->l0.create(desc->l0, ix->d, ix->v, ix->h, false);
+            require(ix->l0.has_centroids, "two-level index requires the meso-cluster centroids");
+        } else {
+            require(ix->l1.has_centroids, "IVF index requires centroids");
+        }
+        ix->rotation.alloc((size_t)ix->d * ix->d);
+        ix->rotation.upload(desc->rotation, (size_t)ix->d * ix->d);
+        std::vector<float> r(ix->d);
+        for (uint32_t i = 0; i < ix->d; ++i) r[i] = ratio_at(i, ix->d);
+        ix->ratios.alloc(ix->d);
+        ix->ratios.upload(r.data(), ix->d);
+        HIP_CHECK(hipDeviceSynchronize());
+        *out = ix.release();
+    });
+}
+
+pdxg_status pdxg_index_destroy(pdxg_index* index) {
+    return guarded([&] {
+        if (!index) return;
+        DeviceGuard g(index->device);
+        delete index;
+    });
+}
+
+size_t pdxg_index_device_bytes(const pdxg_index* ix) {
+    if (!ix) return 0;
+    return ix->l1.device_bytes() + ix->l0.device_bytes() + ix->rotation.bytes() + ix->ratios.bytes();
+}
+
+pdxg_status pdxg_index_update_cluster(pdxg_index* ix, uint32_t level, uint32_t cluster_id,
+                                      const pdxg_cluster_desc* cd) {
+    return guarded([&] {
+        require(ix && cd, "null argument");
+        require(level <= 1, "level must be 0 or 1");
+        require(level == 1 || ix->tree(), "flat indexes have no level 0");
+        DeviceGuard g(ix->device);
+        std::lock_guard<std::mutex> lk(ix->mu);
+        Level& L = level == 1 ? ix->l1 : ix->l0;
+        require(cluster_id < L.nc, "cluster id out of range");
+        require(cd->max_capacity == L.cap[cluster_id], "max_capacity change requires re-creating the index");
+        L.check_cluster(*cd);
+        L.upload_cluster(cluster_id, *cd);
+        L.sync_counts();
+        HIP_CHECK(hipDeviceSynchronize());
+    });
+}
+
+size_t pdxg_workspace_bytes(const pdxg_index* index, uint32_t max_queries, uint32_t max_k, uint32_t max_nprobe) {
+    if (!index) return 0;
+    return workspace_bytes_for(index, max_queries, max_k, max_nprobe);
+}
+
+pdxg_status pdxg_workspace_create(pdxg_index* ix, uint32_t max_queries, uint32_t max_k, uint32_t max_nprobe,
+                                  pdxg_workspace** out) {
+    return guarded([&] {
+        require(ix && out, "null argument");
+        require(max_queries > 0 && max_k > 0, "max_queries and max_k must be positive");
+        *out = nullptr;
+        DeviceGuard g(ix->device);
+        std::unique_ptr<pdxg_workspace> ws(new pdxg_workspace);
+        ws->device = ix->device;
+        plan_workspace(ix, max_queries, max_k, max_nprobe, *ws);
+        alloc_workspace(ix, *ws);
+        *out = ws.release();
+    });
+}
+
+pdxg_status pdxg_workspace_destroy(pdxg_workspace* ws) {
+    return guarded([&] {
+        if (!ws) return;
+        DeviceGuard g(ws->device);
+        delete ws;
+    });
+}
+
+pdxg_status pdxg_search_device(pdxg_index* ix, pdxg_workspace* ws, const pdxg_search_args* args, void* stream) {
+    return guarded([&] {
+        require(ix && ws && args, "null argument");
+        DeviceGuard g(ix->device);
+        search_device(ix, *ws, *args, static_cast<hipStream_t>(stream));
+    });
+}
+
+pdxg_status pdxg_search(pdxg_index* ix, const pdxg_search_args* args) {
+    return guarded([&] {
+        require(ix && args, "null argument");
+        DeviceGuard g(ix->device);
+        std::lock_guard<std::mutex> lk(ix->mu);
+        const pdxg_search_args& a = *args;
+        const uint32_t nq = a.num_queries;
+        require(a.queries || nq == 0, "queries required");
+        uint32_t kmax = a.k_max;
+        if (a.k) {
+            for (uint32_t q = 0; q < nq; ++q) require(a.k[q] >= 1, "k must be >= 1");
+        }
+        require(kmax >= 1, "k_max must be >= 1");
+        if (nq == 0) return;
+        pdxg_workspace* ws = host_workspace(ix, nq, kmax, a.nprobe_max);
+        const size_t d = ix->d;
+        ix->h_queries.ensure((size_t)nq * d);
+        ix->h_queries.upload(a.queries, (size_t)nq * d);
+        pdxg_search_args da = a;
+        da.queries = ix->h_queries.p;
+        if (a.k) {
+            ix->h_k.ensure(nq);
+            ix->h_k.upload(a.k, nq);
+            da.k = ix->h_k.p;
+        }
+        if (a.nprobe) {
+            ix->h_np.ensure(nq);
+            ix->h_np.upload(a.nprobe, nq);
+            da.nprobe = ix->h_np.p;
+        }
+        const size_t no = (size_t)nq * a.out_stride;
+        ix->h_ids.ensure(no);
+        ix->h_dists.ensure(no);
+        ix->h_counts.ensure(nq);
+        da.out_ids = ix->h_ids.p;
+        da.out_distances = ix->h_dists.p;
+        da.out_counts = ix->h_counts.p;
+        da.out_int_distances = nullptr;
+        if (a.out_int_distances) {
+            ix->h_idists.ensure(no);
+            da.out_int_distances = ix->h_idists.p;
+        }
+        da.out_bytes = nullptr;
+        if (a.out_bytes) {
+            ix->h_bytes.ensure(nq);
+            da.out_bytes = reinterpret_cast<uint64_t*>(ix->h_bytes.p);
+        }
+        search_device(ix, *ws, da, nullptr);
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(a.out_counts, ix->h_counts.p, nq * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(a.out_ids, ix->h_ids.p, no * 4, hipMemcpyDeviceToHost));
+        if (a.out_distances) HIP_CHECK(hipMemcpy(a.out_distances, ix->h_dists.p, no * 4, hipMemcpyDeviceToHost));
+        if (a.out_int_distances)
+            HIP_CHECK(hipMemcpy(a.out_int_distances, ix->h_idists.p, no * 4, hipMemcpyDeviceToHost));
+        if (a.out_bytes) HIP_CHECK(hipMemcpy(a.out_bytes, ix->h_bytes.p, nq * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+pdxg_status pdxg_preprocess(pdxg_index* ix, const float* queries, uint32_t nq, uint32_t transformed,
+                            float* out_rot, uint8_t* out_q) {
+    return guarded([&] {
+        require(ix && queries && out_rot, "null argument");
+        DeviceGuard g(ix->device);
+        std::lock_guard<std::mutex> lk(ix->mu);
+        if (nq == 0) return;
+        pdxg_workspace* ws = host_workspace(ix, nq, ix->ws ? ix->ws->max_k : 1u, ix->ws ? ix->ws->max_nprobe : 1u);
+        ix->h_queries.ensure((size_t)nq * ix->d);
+        ix->h_queries.upload(queries, (size_t)nq * ix->d);
+        launch_preprocess(ix, *ws, ix->h_queries.p, nq, transformed != 0, nullptr);
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out_rot, ws->qrot.p, (size_t)nq * ix->d * 4, hipMemcpyDeviceToHost));
+        if (out_q && ix->u8()) HIP_CHECK(hipMemcpy(out_q, ws->qu8.p, (size_t)nq * ix->d, hipMemcpyDeviceToHost));
+    });
+}
+
+pdxg_status pdxg_filtered_search(pdxg_index* ix, const float* query, uint32_t transformed, uint32_t k,
+                                 uint32_t nprobe, const uint8_t* selection, uint32_t* out_ids, float* out_dists,
+                                 uint32_t* out_idists, uint32_t* out_count) {
+    return guarded([&] {
+        require(ix && query && selection && out_ids && out_count, "null argument");
+        require(k >= 1, "k must be >= 1");
+        require(k <= kMaxK, "k too large for the GPU path", PDXG_ERR_UNSUPPORTED);
+        DeviceGuard g(ix->device);
+        std::lock_guard<std::mutex> lk(ix->mu);
+        Level& L = ix->l1;
+        require(L.has_centroids, "filtered search needs the data-level centroids", PDXG_ERR_UNSUPPORTED);
+        pdxg_workspace* ws = host_workspace(ix, 1, std::max(k, ix->ws ? ix->ws->max_k : 1u), 0);
+        ix->h_queries.ensure(ix->d);
+        ix->h_queries.upload(query, ix->d);
+        launch_preprocess(ix, *ws, ix->h_queries.p, 1, transformed != 0, nullptr);
+        // selection vector and per-cluster passing counts (PredicateEvaluator)
+        std::vector<uint32_t> npass(L.nc, 0);
+        for (uint32_t c = 0; c < L.nc; ++c)
+            for (uint32_t i = 0; i < L.cap[c]; ++i) npass[c] += selection[L.slot_off[c] + i] == 1;
+        ix->sel.ensure(L.total_slots);
+        ix->sel.upload(selection, L.total_slots);
+        ix->npass.ensure(L.nc);
+        ix->npass.upload(npass.data(), L.nc);
+        const uint32_t nv = (nprobe == 0 || nprobe > L.nc) ? L.nc : nprobe;
+        std::vector<uint32_t> hk(1, k), hnv(1, nv);
+        ws->k1.upload(hk.data(), 1);
+        ws->nv1.upload(hnv.data(), 1);
+        ix->f_order.ensure(L.nc);
+        ix->f_probe.ensure(L.nc);
+        launch_probe(ix, L, ws->qrot.p, 1, ws->nv1.p, ix->f_order.p, L.nc, ix->f_probe.p, nullptr);
+        ix->h_ids.ensure(k);
+        ix->h_dists.ensure(k);
+        ix->h_idists.ensure(k);
+        ix->h_counts.ensure(1);
+        ScanArgs s{};
+        s.L = L.view(ix->qscale_sq, ix->inv_qscale_sq);
+        s.ratios = ix->ratios.p;
+        s.qf = ws->qrot.p;
+        s.qu = ws->qu8.p;
+        s.order = ix->f_order.p;
+        s.order_stride = L.nc;
+        s.nvisit = ws->nv1.p;
+        s.kq = ws->k1.p;
+        s.out_ids = ix->h_ids.p;
+        s.out_dists = ix->h_dists.p;
+        s.out_idists = ix->h_idists.p;
+        s.out_counts = ix->h_counts.p;
+        s.out_stride = k;
+        s.sel = ix->sel.p;
+        s.n_pass = ix->npass.p;
+        s.nq = 1;
+        if (ix->u8()) launch_scan<true, true>(ix, L, s, k, nullptr);
+        else launch_scan<false, true>(ix, L, s, k, nullptr);
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out_count, ix->h_counts.p, 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(out_ids, ix->h_ids.p, (size_t)*out_count * 4, hipMemcpyDeviceToHost));
+        if (out_dists) HIP_CHECK(hipMemcpy(out_dists, ix->h_dists.p, (size_t)*out_count * 4, hipMemcpyDeviceToHost));
+        if (out_idists) HIP_CHECK(hipMemcpy(out_idists, ix->h_idists.p, (size_t)*out_count * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

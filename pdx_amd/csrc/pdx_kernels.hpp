@@ -1,0 +1,779 @@
+// HIP kernels of the PDX search path for gfx950.
+//
+//   k_setup          per-query k / probe counts          (searcher.hpp:614-627, index.hpp:545-566)
+//   k_normalize      query L2 normalisation              (quantizers/scalar.hpp:23-40)
+//   k_rotate         y = x . M, fused SQ8 quantisation   (pruners/adsampling.hpp:225-245, scalar.hpp:76-94)
+//   k_probe_dist     query x centroid L2 distances       (searcher.hpp:181-195)
+//   k_probe_select   ascending top-nprobe per query      (searcher.hpp:196-214)
+//   k_scan           Start / Warmup / Prune / Merge over a query's ordered clusters
+//                    (searcher.hpp:218-566, 596-856), one workgroup per query
+#pragma once
+#include <float.h>
+
+#include "pdx_device.hpp"
+
+namespace pdxg {
+
+// DIMENSIONS_FETCHING_SIZES, common.hpp:47-49
+__constant__ uint32_t c_fetch[20] = {16,  16,  32,  32,  32,  32,  64,   64,   64,   64,
+                                     128, 128, 128, 128, 256, 256, 512, 1024, 2048, 16384};
+
+// Device view of one IVF level (IVF<Q>, ivf_wrapper.hpp:15-179). All clusters live in one pool;
+// cluster c's PDX buffer starts at data + data_off[c] (bytes) with stride cap[c] vectors.
+struct LevelDev {
+    const uint8_t* data;
+    const uint64_t* data_off;
+    const uint32_t* ids;       // slot row ids, cluster c at slot_off[c]
+    const uint64_t* slot_off;  // running sum of max_capacity (cluster_offsets)
+    const uint32_t* n_used;
+    const uint32_t* n_live;
+    const uint32_t* cap;
+    const uint32_t* tomb;      // tombstone bitmap, cluster c at tomb_off[c] (words)
+    const uint64_t* tomb_off;
+    const uint32_t* n_tomb;
+    const float* centroids;    // nc x d
+    uint32_t nc, d, v, h;
+    float scale_sq, inv_scale_sq;
+};
+
+struct ScanArgs {
+    LevelDev L;
+    const float* ratios;        // ADSampling ratio table [d]
+    const float* qf;            // F32 queries [nq x d]
+    const uint8_t* qu;          // U8 queries [nq x d]
+    const uint32_t* order;      // cluster visit order [nq x order_stride]
+    uint32_t order_stride;
+    const uint32_t* nvisit;     // clusters to visit per query
+    const uint32_t* order_count;  // optional per-query cap on nvisit (length of `order` rows)
+    const uint32_t* kq;         // k per query
+    uint32_t* out_ids;
+    float* out_dists;
+    uint32_t* out_idists;
+    uint32_t* out_counts;
+    uint32_t out_stride;
+    unsigned long long* out_bytes;
+    const uint8_t* sel;         // filtered search: selection byte per slot (cluster at slot_off)
+    const uint32_t* n_pass;     // filtered search: passing slots per cluster
+    uint32_t nq;
+    // dynamic LDS layout (byte offsets)
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red;
+};
+
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t* s_red) {
+    constexpr uint32_t NW = BLOCK / kWave;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) s_red[w] = x;
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < NW; ++i) tot += s_red[i];
+    __syncthreads();
+    return tot;
+}
+
+// ------------------------------------------------------------------------------------------
+// Per-query parameters
+// ------------------------------------------------------------------------------------------
+struct SetupArgs {
+    const uint32_t* k_in;
+    const uint32_t* nprobe_in;
+    uint32_t k_all, nprobe_all, nq;
+    uint32_t tree;
+    uint32_t nc1, nc0;
+    uint32_t k_cap;      // largest k the scan buffers hold
+    uint32_t order_cap;  // flat: row length of the probe order buffer
+    uint32_t k0_cap;     // tree: row length of the L0 result buffer
+    uint32_t* k1;        // final k
+    uint32_t* nvisit1;   // data-level clusters to visit
+    uint32_t* k0;        // tree: L0 search k (= nprobe)
+    uint32_t* nvisit0;   // tree: meso clusters to visit
+    unsigned long long* bytes;      // optional per-query byte counters, initialised here
+    unsigned long long probe_bytes; // centroid bytes streamed by the first probe
+};
+
+__global__ void k_setup(SetupArgs a) {
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.nq) return;
+    // Values outside the declared maxima are clamped so that no buffer can be overrun.
+    uint32_t k = a.k_in ? a.k_in[q] : a.k_all;
+    k = k < 1 ? 1 : (k > a.k_cap ? a.k_cap : k);
+    uint32_t np = a.nprobe_in ? a.nprobe_in[q] : a.nprobe_all;
+    a.k1[q] = k;
+    if (a.bytes) a.bytes[q] = a.probe_bytes;
+    if (!a.tree) {  // searcher.hpp:614-616
+        uint32_t nv = (np == 0 || np > a.nc1) ? a.nc1 : np;
+        a.nvisit1[q] = nv > a.order_cap ? a.order_cap : nv;
+        return;
+    }
+    // index.hpp:545-566: nprobe 0 means all L1 clusters; halve the meso probes when visiting
+    // fewer than half of the L1 clusters; the L0 search returns the top-nprobe L1 ids.
+    uint32_t np1 = np == 0 ? a.nc1 : np;
+    uint32_t np0 = a.nc0;
+    if (np1 < a.nc1 / 2) np0 /= 2;
+    a.k0[q] = np1 > a.k0_cap ? a.k0_cap : np1;
+    a.nvisit0[q] = (np0 == 0 || np0 > a.nc0) ? a.nc0 : np0;
+    a.nvisit1[q] = np1 > a.nc1 ? a.nc1 : np1;
+}
+
+// ------------------------------------------------------------------------------------------
+// Query preprocessing
+// ------------------------------------------------------------------------------------------
+// One workgroup per query; the 16 lanes of row 0 sum squares in the 16-lane order.
+__global__ void k_normalize(const float* __restrict__ x, float* __restrict__ out, uint32_t d, uint32_t nq) {
+    __shared__ float s_norm;
+    __shared__ int s_zero;
+    const uint32_t q = blockIdx.x;
+    if (q >= nq) return;
+    const float* xq = x + (size_t)q * d;
+    if (threadIdx.x < 16) {
+        float acc = 0.0f;
+        for (uint32_t j = threadIdx.x; j < d; j += 16) acc = __builtin_fmaf(xq[j], xq[j], acc);
+        acc = tree16(acc);
+        if (threadIdx.x == 0) {
+            s_zero = acc == 0.0f;
+            s_norm = __builtin_sqrtf(acc);
+        }
+    }
+    __syncthreads();
+    const float nrm = s_norm;
+    const bool zero = s_zero;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) out[(size_t)q * d + j] = zero ? xq[j] : xq[j] / nrm;
+}
+
+// y[q][i] = sum_j x[q][j] * M[j][i] as one fma chain in j order per output; optional SQ8
+// quantisation of y: clamp(round((y - base) * scale), 0, 255) with round-half-away-from-zero.
+constexpr uint32_t kRotTQ = 16;   // queries per workgroup
+constexpr uint32_t kRotTI = 64;   // outputs per workgroup
+constexpr uint32_t kRotTJ = 64;   // staged input dims
+__global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ x, const float* __restrict__ M,
+                                                float* __restrict__ y, uint8_t* __restrict__ yq, uint32_t d,
+                                                uint32_t nq, float base, float scale, int identity) {
+    __shared__ float xs[kRotTQ][kRotTJ + 1];
+    const uint32_t tx = threadIdx.x % kRotTI, ty = threadIdx.x / kRotTI;  // ty in 0..3
+    const uint32_t i = blockIdx.x * kRotTI + tx;
+    const uint32_t q0 = blockIdx.y * kRotTQ;
+    float acc[kRotTQ / 4];
+#pragma unroll
+    for (uint32_t r = 0; r < kRotTQ / 4; ++r) acc[r] = 0.0f;
+    if (!identity) {
+        for (uint32_t j0 = 0; j0 < d; j0 += kRotTJ) {
+            for (uint32_t t = threadIdx.x; t < kRotTQ * kRotTJ; t += 256) {
+                uint32_t qq = t / kRotTJ, jj = t % kRotTJ;
+                uint32_t q = q0 + qq, j = j0 + jj;
+                xs[qq][jj] = (q < nq && j < d) ? x[(size_t)q * d + j] : 0.0f;
+            }
+            __syncthreads();
+            const uint32_t jl = min(kRotTJ, d - j0);
+            if (i < d) {
+                for (uint32_t jj = 0; jj < jl; ++jj) {
+                    float m = M[(size_t)(j0 + jj) * d + i];
+#pragma unroll
+                    for (uint32_t r = 0; r < kRotTQ / 4; ++r) acc[r] = __builtin_fmaf(xs[ty + 4 * r][jj], m, acc[r]);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (i >= d) return;
+#pragma unroll
+    for (uint32_t r = 0; r < kRotTQ / 4; ++r) {
+        uint32_t q = q0 + ty + 4 * r;
+        if (q >= nq) continue;
+        float v = identity ? x[(size_t)q * d + i] : acc[r];
+        if (y) y[(size_t)q * d + i] = v;
+        if (yq) {
+            float t = (v - base) * scale;
+            int rr = static_cast<int>(roundf(t));
+            yq[(size_t)q * d + i] = static_cast<uint8_t>(rr > 255 ? 255 : (rr < 0 ? 0 : rr));
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// IVF probe
+// ------------------------------------------------------------------------------------------
+// dist[q][c] = l2sq16(q, centroid c). One workgroup per (query, 64 centroids); each 16-lane row
+// computes one centroid at a time.
+constexpr uint32_t kProbeCPB = 64;
+__global__ __launch_bounds__(256) void k_probe_dist(const float* __restrict__ qf, const float* __restrict__ cent,
+                                                    float* __restrict__ out, uint32_t d, uint32_t nc, uint32_t nq) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    float* qs = reinterpret_cast<float*>(lds_raw);
+    const uint32_t q = blockIdx.x;
+    if (q >= nq) return;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) qs[j] = qf[(size_t)q * d + j];
+    __syncthreads();
+    const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16, G = blockDim.x / 16;
+    const uint32_t c0 = blockIdx.y * kProbeCPB;
+    for (uint32_t cc = g; cc < kProbeCPB; cc += G) {
+        uint32_t c = c0 + cc;
+        if (c >= nc) break;  // uniform across the 16-lane row
+        const float* cr = cent + (size_t)c * d;
+        float acc = 0.0f;
+        for (uint32_t j = l; j < d; j += 16) {
+            float df = qs[j] - cr[j];
+            acc = __builtin_fmaf(df, df, acc);
+        }
+        acc = tree16(acc);
+        if (l == 0) out[(size_t)q * nc + c] = acc;
+    }
+}
+
+// First nvisit[q] cluster ids by ascending (distance, id). Bitonic sort of n2 >= nc keys in LDS.
+__global__ __launch_bounds__(256) void k_probe_select(const float* __restrict__ dist, uint32_t nc, uint32_t n2,
+                                                      const uint32_t* __restrict__ nvisit, uint32_t* __restrict__ order,
+                                                      uint32_t order_stride, uint32_t nq) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    uint64_t* keys = reinterpret_cast<uint64_t*>(lds_raw);
+    const uint32_t q = blockIdx.x;
+    if (q >= nq) return;
+    for (uint32_t c = threadIdx.x; c < n2; c += 256)
+        keys[c] = c < nc ? make_key(dist[(size_t)q * nc + c], c) : kKeyInf;
+    __syncthreads();
+    bitonic_sort<256>(keys, nullptr, n2);
+    const uint32_t nv = nvisit[q];
+    for (uint32_t t = threadIdx.x; t < nv; t += 256) order[(size_t)q * order_stride + t] = key_id(keys[t]);
+}
+
+// ------------------------------------------------------------------------------------------
+// The pruned PDX scan
+// ------------------------------------------------------------------------------------------
+template <bool U8>
+struct DistT {
+    using type = float;
+};
+template <>
+struct DistT<true> {
+    using type = uint32_t;
+};
+
+constexpr uint32_t kVT = 4;  // vectors per thread per vertical tile
+
+template <bool U8, bool FILT, uint32_t BLOCK>
+struct Scanner {
+    using dist_t = typename DistT<U8>::type;
+    static constexpr uint32_t G = BLOCK / 16;  // 16-lane rows per workgroup
+
+    const ScanArgs& a;
+    uint8_t* lds;
+    float* qf;        // F32 query [d]
+    uint32_t* qw;     // U8 query words: [nvw] vertical then [h/4] horizontal
+    uint32_t* qq;     // U8 per-word sum of q^2
+    dist_t* dist;
+    uint32_t* pos;
+    uint64_t *H, *T, *O, *CK;
+    uint32_t *HP, *TP, *OP, *CP, *red;
+    uint32_t d, v, h, nvw;
+    uint32_t hl = 0;
+    unsigned long long bytes = 0;
+
+    // per-cluster state
+    const uint8_t* base;
+    const uint32_t* cid;
+    const uint32_t* tbm;
+    const uint8_t* csel;
+    uint32_t n, S, ntomb;
+
+    __device__ Scanner(const ScanArgs& args, uint8_t* l) : a(args), lds(l) {
+        qf = reinterpret_cast<float*>(lds + a.o_q);
+        qw = reinterpret_cast<uint32_t*>(lds + a.o_q);
+        qq = reinterpret_cast<uint32_t*>(lds + a.o_qq);
+        dist = reinterpret_cast<dist_t*>(lds + a.o_dist);
+        pos = reinterpret_cast<uint32_t*>(lds + a.o_pos);
+        H = reinterpret_cast<uint64_t*>(lds + a.o_h);
+        HP = reinterpret_cast<uint32_t*>(lds + a.o_hp);
+        T = reinterpret_cast<uint64_t*>(lds + a.o_t);
+        TP = reinterpret_cast<uint32_t*>(lds + a.o_tp);
+        O = reinterpret_cast<uint64_t*>(lds + a.o_o);
+        OP = reinterpret_cast<uint32_t*>(lds + a.o_op);
+        CK = reinterpret_cast<uint64_t*>(lds + a.o_ck);
+        CP = reinterpret_cast<uint32_t*>(lds + a.o_cp);
+        red = reinterpret_cast<uint32_t*>(lds + a.o_red);
+        d = a.L.d;
+        v = a.L.v;
+        h = a.L.h;
+        nvw = (v + 3) / 4;
+    }
+
+    static __device__ __forceinline__ dist_t mask_value() {
+        if constexpr (U8) return 0xFFFFFFFFu / 2u;
+        else return FLT_MAX / 2.0f;
+    }
+
+    __device__ __forceinline__ float to_float(dist_t x) const {
+        if constexpr (U8) return static_cast<float>(x) * a.L.inv_scale_sq;
+        else return x;
+    }
+
+    __device__ __forceinline__ bool tombstoned(uint32_t i) const {
+        return ntomb != 0 && ((tbm[i >> 5] >> (i & 31)) & 1u);
+    }
+
+    // GetPruningThreshold: adsampling.hpp:91-98 with the U8 conversion of searcher.hpp:75-91.
+    __device__ __forceinline__ dist_t threshold(uint32_t k, uint32_t cur) const {
+        float top = key_dist(H[k - 1]);
+        float ratio = cur == d ? 1.0f : a.ratios[cur];
+        float tf = top * ratio;
+        if constexpr (U8) {
+            float sc = tf * a.L.scale_sq;
+            return sc >= 4294967296.0f ? 0xFFFFFFFFu : static_cast<uint32_t>(sc);
+        } else {
+            return tf;
+        }
+    }
+
+    __device__ __forceinline__ uint32_t qbyte(uint32_t j) const { return (qw[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
+
+    __device__ void load_query(uint32_t q) {
+        if constexpr (!U8) {
+            for (uint32_t j = threadIdx.x; j < d; j += BLOCK) qf[j] = a.qf[(size_t)q * d + j];
+        } else {
+            const uint8_t* src = a.qu + (size_t)q * d;
+            for (uint32_t w = threadIdx.x; w < nvw; w += BLOCK) {
+                uint32_t word = 0;
+                for (uint32_t t = 0; t < 4; ++t) {
+                    uint32_t j = 4 * w + t;
+                    word |= (j < v ? static_cast<uint32_t>(src[j]) : 0u) << (8 * t);
+                }
+                qw[w] = word;
+                qq[w] = sumsq4(word);
+            }
+            for (uint32_t w = threadIdx.x; w < h / 4; w += BLOCK) {
+                uint32_t word = 0;
+                for (uint32_t t = 0; t < 4; ++t) word |= static_cast<uint32_t>(src[v + 4 * w + t]) << (8 * t);
+                qw[nvw + w] = word;
+                qq[nvw + w] = sumsq4(word);
+            }
+        }
+        __syncthreads();
+    }
+
+    // Vertical (scalar_computers.hpp:19-44 / 77-121) over dims [da, db) for the VT slots
+    // i0 + t*BLOCK + tid of this thread; slots >= n are clamped to a valid slot and ignored.
+    __device__ __forceinline__ void vertical_tile(dist_t (&acc)[kVT], uint32_t i0, uint32_t da, uint32_t db) const {
+        uint32_t idx[kVT];
+#pragma unroll
+        for (uint32_t t = 0; t < kVT; ++t) idx[t] = min(i0 + t * BLOCK + threadIdx.x, n - 1);
+        if constexpr (!U8) {
+            const float* data = reinterpret_cast<const float*>(base);
+            uint32_t j = da;
+            for (; j + 4 <= db; j += 4) {
+                float x[4][kVT];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const float qj = qf[j + u];
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t) {
+                        float df = qj - x[u][t];
+                        acc[t] = __builtin_fmaf(df, df, acc[t]);
+                    }
+                }
+            }
+            for (; j < db; ++j) {
+                const float qj = qf[j];
+#pragma unroll
+                for (uint32_t t = 0; t < kVT; ++t) {
+                    float df = qj - data[(size_t)j * S + idx[t]];
+                    acc[t] = __builtin_fmaf(df, df, acc[t]);
+                }
+            }
+        } else {
+            uint32_t j = da;
+            for (; j + 16 <= db; j += 16) {
+                uint32_t x[4][kVT];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t)
+                        x[u][t] = *reinterpret_cast<const uint32_t*>(base + (size_t)(j + 4 * u) * S + 4 * idx[t]);
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t w = (j >> 2) + u;
+                    const uint32_t qv = qw[w], qs = qq[w];
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
+                }
+            }
+            for (; j + 4 <= db; j += 4) {
+                const uint32_t w = j >> 2;
+                const uint32_t qv = qw[w], qs = qq[w];
+#pragma unroll
+                for (uint32_t t = 0; t < kVT; ++t)
+                    acc[t] += sqdist4(qv, *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * idx[t]), qs);
+            }
+            if (j < db) {  // tail group of rem < 4 dims, stored as rem bytes per vector
+                const uint32_t rem = db - j;
+#pragma unroll
+                for (uint32_t t = 0; t < kVT; ++t)
+                    for (uint32_t u = 0; u < rem; ++u) {
+                        int df = static_cast<int>(qbyte(j + u)) -
+                                 static_cast<int>(base[(size_t)j * S + (size_t)idx[t] * rem + u]);
+                        acc[t] += static_cast<uint32_t>(df * df);
+                    }
+            }
+        }
+    }
+
+    // Vertical over dims [da, db) for one gathered slot i (VerticalPruning).
+    __device__ __forceinline__ dist_t vertical_one(dist_t acc, uint32_t i, uint32_t da, uint32_t db) const {
+        if constexpr (!U8) {
+            const float* data = reinterpret_cast<const float*>(base);
+            for (uint32_t j = da; j < db; ++j) {
+                float df = qf[j] - data[(size_t)j * S + i];
+                acc = __builtin_fmaf(df, df, acc);
+            }
+        } else {
+            uint32_t j = da;
+            for (; j + 4 <= db; j += 4) {
+                const uint32_t w = j >> 2;
+                acc += sqdist4(qw[w], *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * i), qq[w]);
+            }
+            if (j < db) {
+                const uint32_t rem = db - j;
+                for (uint32_t u = 0; u < rem; ++u) {
+                    int df = static_cast<int>(qbyte(j + u)) - static_cast<int>(base[(size_t)j * S + (size_t)i * rem + u]);
+                    acc += static_cast<uint32_t>(df * df);
+                }
+            }
+        }
+        return acc;
+    }
+
+    // Horizontal 64-dim block `hd` of slot i, computed by the 16 lanes of this thread's row
+    // (valid in lane 0 of the row).
+    __device__ __forceinline__ dist_t horizontal16(uint32_t i, uint32_t hd, uint32_t l) const {
+        if constexpr (!U8) {
+            const float* blk = reinterpret_cast<const float*>(base) + (size_t)(v + hd) * S + (size_t)i * kHDim;
+            const float* qh = qf + v + hd;
+            float x0 = blk[l], x1 = blk[l + 16], x2 = blk[l + 32], x3 = blk[l + 48];
+            float acc = 0.0f, df;
+            df = qh[l] - x0;
+            acc = __builtin_fmaf(df, df, acc);
+            df = qh[l + 16] - x1;
+            acc = __builtin_fmaf(df, df, acc);
+            df = qh[l + 32] - x2;
+            acc = __builtin_fmaf(df, df, acc);
+            df = qh[l + 48] - x3;
+            acc = __builtin_fmaf(df, df, acc);
+            return tree16(acc);
+        } else {
+            const uint8_t* blk = base + (size_t)(v + hd) * S + (size_t)i * kHDim;
+            const uint32_t w = nvw + hd / 4 + l;
+            uint32_t x = *reinterpret_cast<const uint32_t*>(blk + 4 * l);
+            return sum16_u(sqdist4(qw[w], x, qq[w]));
+        }
+    }
+
+    __device__ __forceinline__ uint64_t key_of(uint32_t i) const {
+        return make_key(to_float(dist[i]), cid[i]);
+    }
+    __device__ __forceinline__ uint32_t pay_of(uint32_t i) const {
+        if constexpr (U8) return dist[i];
+        else return 0u;
+    }
+    __device__ __forceinline__ bool selected(uint32_t i) const {
+        if constexpr (FILT) return csel[i] == 1;
+        else return true;
+    }
+
+    // Push the m smallest slots (all n, keys (distance, id)) onto the heap: the partial_sort
+    // + push of Start / FilteredStart (searcher.hpp:256-280, 347-371).
+    __device__ void push_m_smallest(uint32_t m) {
+        uint32_t tl = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
+            uint32_t i = i0 + threadIdx.x;
+            bool valid = i < n;
+            uint64_t key = valid ? key_of(i) : kKeyInf;
+            uint32_t pay = valid ? pay_of(i) : 0u;
+            tl = list_insert_chunk<BLOCK>(valid, key, pay, T, TP, tl, m, CK, CP, O, OP, red);
+        }
+        uint32_t nl = merge_sorted<BLOCK>(H, HP, hl, T, TP, tl, O, OP, hl + tl);
+        for (uint32_t i = threadIdx.x; i < nl; i += BLOCK) {
+            H[i] = O[i];
+            HP[i] = OP[i];
+        }
+        __syncthreads();
+        hl = nl;
+        bytes += 4ull * m;
+    }
+
+    __device__ void mask_tombstones_all() {
+        if (ntomb == 0) return;
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
+            if (tombstoned(i)) dist[i] = mask_value();
+        __syncthreads();
+    }
+
+    // searcher.hpp:218-281
+    __device__ void start(uint32_t k) {
+        for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
+            dist_t acc[kVT];
+#pragma unroll
+            for (uint32_t t = 0; t < kVT; ++t) acc[t] = dist_t(0);
+            vertical_tile(acc, i0, 0, v);
+#pragma unroll
+            for (uint32_t t = 0; t < kVT; ++t) {
+                uint32_t i = i0 + t * BLOCK + threadIdx.x;
+                if (i < n) dist[i] = acc[t];
+            }
+        }
+        __syncthreads();
+        if (h) {
+            const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16;
+            for (uint32_t i = g; i < n; i += G) {
+                dist_t dd = dist[i];
+                for (uint32_t hd = 0; hd < h; hd += kHDim) dd += horizontal16(i, hd, l);
+                if (l == 0) dist[i] = dd;
+            }
+            __syncthreads();
+        }
+        bytes += (unsigned long long)n * d * (U8 ? 1 : 4);
+        mask_tombstones_all();
+        uint32_t m = min(k - hl, n);
+        push_m_smallest(m);
+    }
+
+    // searcher.hpp:284-372
+    __device__ void filtered_start(uint32_t k, uint32_t passing) {
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) dist[i] = dist_t(0);
+        uint32_t np = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
+            uint32_t i = i0 + threadIdx.x;
+            bool p = i < n && selected(i);
+            uint32_t s;
+            uint32_t r = block_excl_scan<BLOCK>(p, red, &s);
+            if (p) pos[np + r] = i;
+            np += s;
+        }
+        __syncthreads();
+        const float selection = static_cast<float>(passing) / static_cast<float>(n);
+        if (h) {
+            const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16;
+            for (uint32_t j = g; j < np; j += G) {
+                uint32_t i = pos[j];
+                dist_t dd = dist[i];
+                for (uint32_t hd = 0; hd < h; hd += kHDim) dd += horizontal16(i, hd, l);
+                if (l == 0) dist[i] = dd;
+            }
+            __syncthreads();
+            bytes += (unsigned long long)np * h * (U8 ? 1 : 4);
+        }
+        if (selection > (1.0f - 0.8f)) {
+            for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
+                dist_t acc[kVT];
+#pragma unroll
+                for (uint32_t t = 0; t < kVT; ++t) {
+                    uint32_t i = min(i0 + t * BLOCK + threadIdx.x, n - 1);
+                    acc[t] = dist[i];
+                }
+                vertical_tile(acc, i0, 0, v);
+#pragma unroll
+                for (uint32_t t = 0; t < kVT; ++t) {
+                    uint32_t i = i0 + t * BLOCK + threadIdx.x;
+                    if (i < n) dist[i] = acc[t];
+                }
+            }
+            bytes += (unsigned long long)n * v * (U8 ? 1 : 4);
+        } else {
+            for (uint32_t j = threadIdx.x; j < np; j += BLOCK) {
+                uint32_t i = pos[j];
+                dist[i] = vertical_one(dist[i], i, 0, v);
+            }
+            bytes += (unsigned long long)np * v * (U8 ? 1 : 4);
+        }
+        __syncthreads();
+        uint32_t m = min(k - hl, passing);
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK)
+            if (!selected(i) || tombstoned(i)) dist[i] = mask_value();
+        __syncthreads();
+        push_m_smallest(m);
+    }
+
+    // searcher.hpp:376-435. Returns the dimension index reached.
+    __device__ uint32_t warmup(uint32_t k, uint32_t passing) {
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+            dist_t x = dist_t(0);
+            if (tombstoned(i)) x = mask_value();
+            if constexpr (FILT) {
+                if (!selected(i)) x = mask_value();
+            }
+            dist[i] = x;
+        }
+        __syncthreads();
+        if constexpr (FILT) {
+            const float selection = static_cast<float>(passing) / static_cast<float>(n);
+            if (selection < (1.0f - 0.8f)) return 0;
+        }
+        const uint32_t need = static_cast<uint32_t>(ceilf(0.8f * static_cast<float>(n)));
+        uint32_t cur = 0, t = 0, npruned = 0;
+        while (npruned < need && cur < v) {
+            const uint32_t end = min(cur + c_fetch[t], v);
+            const dist_t tau = threshold(k, end);
+            uint32_t cnt = 0;
+            for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
+                dist_t acc[kVT];
+#pragma unroll
+                for (uint32_t u = 0; u < kVT; ++u) acc[u] = dist[min(i0 + u * BLOCK + threadIdx.x, n - 1)];
+                vertical_tile(acc, i0, cur, end);
+#pragma unroll
+                for (uint32_t u = 0; u < kVT; ++u) {
+                    uint32_t i = i0 + u * BLOCK + threadIdx.x;
+                    if (i < n) {
+                        dist[i] = acc[u];
+                        cnt += acc[u] >= tau;
+                    }
+                }
+            }
+            npruned = block_sum<BLOCK>(cnt, red);
+            bytes += (unsigned long long)n * (end - cur) * (U8 ? 1 : 4);
+            cur = end;
+            ++t;
+        }
+        return cur;
+    }
+
+    // Stable in-place filter of pos[0..np) by dist < tau (EvaluatePruningPredicateOnPositionsArray).
+    __device__ uint32_t filter(uint32_t np, dist_t tau) {
+        uint32_t out = 0;
+        for (uint32_t j0 = 0; j0 < np; j0 += BLOCK) {
+            uint32_t j = j0 + threadIdx.x;
+            uint32_t pj = j < np ? pos[j] : 0u;
+            bool p = j < np && dist[pj] < tau;
+            uint32_t s;
+            uint32_t r = block_excl_scan<BLOCK>(p, red, &s);
+            if (p) pos[out + r] = pj;
+            out += s;
+        }
+        __syncthreads();
+        return out;
+    }
+
+    // searcher.hpp:438-540. Returns the number of survivors left in pos[].
+    __device__ uint32_t prune(uint32_t k, uint32_t cur) {
+        dist_t tau = threshold(k, cur);
+        uint32_t np = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
+            uint32_t i = i0 + threadIdx.x;
+            bool valid = i < n;
+            dist_t x = dist_t(0);
+            if (valid) {
+                x = dist[i];
+                if (tombstoned(i)) {
+                    x = mask_value();
+                    dist[i] = x;
+                }
+            }
+            bool p = valid && x < tau && selected(i);
+            uint32_t s;
+            uint32_t r = block_excl_scan<BLOCK>(p, red, &s);
+            if (p) pos[np + r] = i;
+            np += s;
+        }
+        __syncthreads();
+        uint32_t cv = cur, hd = 0;
+        const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16;
+        while (h && np && hd < h) {
+            for (uint32_t j = g; j < np; j += G) {
+                uint32_t i = pos[j];
+                dist_t hv = horizontal16(i, hd, l);
+                if (l == 0) dist[i] += hv;
+            }
+            __syncthreads();
+            bytes += (unsigned long long)np * kHDim * (U8 ? 1 : 4);
+            hd += kHDim;
+            cur += kHDim;
+            tau = threshold(k, cur);
+            np = filter(np, tau);
+        }
+        while (np && cv < v) {
+            const uint32_t end = min(cv + kHDim, v);
+            for (uint32_t j = threadIdx.x; j < np; j += BLOCK) {
+                uint32_t i = pos[j];
+                dist[i] = vertical_one(dist[i], i, cv, end);
+            }
+            __syncthreads();
+            bytes += (unsigned long long)np * (end - cv) * (U8 ? 1 : 4);
+            cur = min(cur + kHDim, d);
+            cv = end;
+            tau = threshold(k, cur);
+            np = filter(np, tau);
+            if (cur == d) break;
+        }
+        return np;
+    }
+
+    // searcher.hpp:542-566
+    __device__ void merge(uint32_t k, uint32_t np) {
+        for (uint32_t j0 = 0; j0 < np; j0 += BLOCK) {
+            uint32_t j = j0 + threadIdx.x;
+            bool valid = j < np;
+            uint32_t i = valid ? pos[j] : 0u;
+            uint64_t key = valid ? key_of(i) : kKeyInf;
+            uint32_t pay = valid ? pay_of(i) : 0u;
+            hl = list_insert_chunk<BLOCK>(valid, key, pay, H, HP, hl, k, CK, CP, O, OP, red);
+        }
+        bytes += 4ull * np;
+    }
+
+    __device__ void run(uint32_t q) {
+        load_query(q);
+        const uint32_t k = a.kq[q];
+        uint32_t nvis = a.nvisit[q];
+        if (a.order_count) nvis = min(nvis, a.order_count[q]);
+        const uint32_t* ord = a.order + (size_t)q * a.order_stride;
+        for (uint32_t ci = 0; ci < nvis; ++ci) {
+            const uint32_t c = ord[ci];
+            uint32_t passing = 0;
+            if constexpr (FILT) {
+                passing = a.n_pass[c];
+                if (passing == 0) continue;
+                csel = a.sel + a.L.slot_off[c];
+            }
+            if (a.L.n_live[c] == 0) continue;
+            n = a.L.n_used[c];
+            S = a.L.cap[c];
+            base = a.L.data + a.L.data_off[c];
+            cid = a.L.ids + a.L.slot_off[c];
+            ntomb = a.L.n_tomb[c];
+            tbm = a.L.tomb + a.L.tomb_off[c];
+            if (hl < k) {
+                if constexpr (FILT) filtered_start(k, passing);
+                else start(k);
+                continue;
+            }
+            uint32_t cur = warmup(k, passing);
+            uint32_t np = prune(k, cur);
+            if (np) merge(k, np);
+        }
+        // searcher.hpp:568-585
+        uint32_t cnt = hl;
+        if (cnt > 0 && key_dist(H[cnt - 1]) == FLT_MAX) --cnt;
+        cnt = min(cnt, k);
+        for (uint32_t i = threadIdx.x; i < cnt; i += BLOCK) {
+            a.out_ids[(size_t)q * a.out_stride + i] = key_id(H[i]);
+            if (a.out_dists) a.out_dists[(size_t)q * a.out_stride + i] = key_dist(H[i]);
+            if (a.out_idists) a.out_idists[(size_t)q * a.out_stride + i] = HP[i];
+        }
+        if (threadIdx.x == 0) {
+            a.out_counts[q] = cnt;
+            if (a.out_bytes) a.out_bytes[q] += bytes;
+        }
+    }
+};
+
+template <bool U8, bool FILT, uint32_t BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    if (blockIdx.x >= a.nq) return;
+    Scanner<U8, FILT, BLOCK> s(a, lds_raw);
+    s.run(blockIdx.x);
+}
+
+}  // namespace pdxg

@@ -1,0 +1,98 @@
+"""HIP path vs the CPU oracle on the same index files and queries (bit-exact).
+
+F32 distances are compared bit-for-bit: both sides use the reference's AVX-512 reduction order
+for horizontal blocks / centroid distances and one fma chain in dimension order for vertical
+blocks. SQ8 integer distances and ids are exact by construction.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleIndex
+from pdx_amd import pdxfile as pf
+from tests.pdx_testdata import index_file
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("pdx_f32", 3000, 128, 24, False),
+    ("pdx_u8", 3000, 128, 24, False),
+    ("pdx_tree_f32", 3000, 128, 24, False),
+    ("pdx_tree_u8", 3000, 128, 24, False),
+    ("pdx_tree_u8", 3000, 128, 24, True),     # normalised (cosine / ip)
+    ("pdx_f32", 2000, 100, 16, False),        # d <= 128, h = 0: no horizontal block
+    ("pdx_u8", 2000, 130, 16, False),         # v = 2: U8 vertical tail, unaligned horizontal query
+    ("pdx_u8", 1500, 33, 12, False),          # d <= 64: all vertical, tail of 1 dim
+    ("pdx_f32", 1500, 64, 12, False),
+    ("pdx_f32", 2500, 384, 20, True),
+    ("pdx_u8", 2500, 384, 20, False),
+    ("pdx_tree_u8", 4000, 768, 40, False),
+    ("pdx_tree_f32", 4000, 768, 40, True),
+]
+
+
+def _ids(c):
+    return "%s-n%d-d%d-c%d%s" % (c[0], c[1], c[2], c[3], "-norm" if c[4] else "")
+
+
+@pytest.fixture(scope="module")
+def gpu_lib():
+    from pdx_amd import _capi
+
+    print("device:", _capi.device_info(0), flush=True)
+    return _capi
+
+
+def _compare(o_res, g_res, u8):
+    oi, od, oid, oc, ob = o_res
+    gi, gd, gid, gc, gb = g_res
+    np.testing.assert_array_equal(gc, oc)
+    for q in range(len(oc)):
+        n = int(oc[q])
+        np.testing.assert_array_equal(gi[q, :n], oi[q, :n], err_msg=f"ids q={q}")
+        np.testing.assert_array_equal(gd[q, :n].view(np.uint32), od[q, :n].view(np.uint32), err_msg=f"dists q={q}")
+        if u8:
+            np.testing.assert_array_equal(gid[q, :n], oid[q, :n], err_msg=f"int dists q={q}")
+    if gb is not None:
+        np.testing.assert_array_equal(gb, ob, err_msg="algorithmic bytes")
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_search_matches_oracle(gpu_lib, case):
+    kind, n, d, nlist, norm = case
+    path, x, q = index_file(kind, n, d, nlist, normalize=norm)
+    o = OracleIndex(path)
+    g = gpu_lib.GpuIndex(pf.load(path))
+    for k in (1, 10, 50):
+        for nprobe in (1, 4, 0):
+            print(f"{_ids(case)} k={k} nprobe={nprobe}", flush=True)
+            o_res = o.search(q, k, nprobe)
+            g_res = g.search(q, k, nprobe, want_bytes=True)
+            _compare(o_res, g_res, o.is_u8)
+
+
+@pytest.mark.parametrize("case", CASES[:5], ids=_ids)
+def test_preprocess_matches_oracle(gpu_lib, case):
+    kind, n, d, nlist, norm = case
+    path, x, q = index_file(kind, n, d, nlist, normalize=norm)
+    o = OracleIndex(path)
+    g = gpu_lib.GpuIndex(pf.load(path))
+    rot, qu = g.preprocess(q)
+    np.testing.assert_array_equal(rot.view(np.uint32), o.preprocess(q).view(np.uint32))
+
+
+def test_mixed_k_and_nprobe_per_query(gpu_lib):
+    path, x, q = index_file("pdx_tree_u8", 4000, 768, 40)
+    o = OracleIndex(path)
+    g = gpu_lib.GpuIndex(pf.load(path))
+    rng = np.random.default_rng(3)
+    k = rng.choice([1, 10, 20, 100], len(q)).astype(np.uint32)
+    nprobe = rng.integers(1, 41, len(q)).astype(np.uint32)
+    _compare(o.search(q, k, nprobe), g.search(q, k, nprobe, want_bytes=True), True)
+
+
+def test_pre_rotated_queries(gpu_lib):
+    path, x, q = index_file("pdx_u8", 2500, 384, 20)
+    o = OracleIndex(path)
+    g = gpu_lib.GpuIndex(pf.load(path))
+    qr = o.preprocess(q)
+    _compare(o.search(qr, 10, 5, transformed=True), g.search(qr, 10, 5, transformed=True, want_bytes=True), True)
