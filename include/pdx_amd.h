@@ -139,6 +139,14 @@ pdxg_status pdxg_filtered_search(pdxg_index* index, const float* query, uint32_t
                                  uint32_t* out_ids, float* out_distances, uint32_t* out_int_distances,
                                  uint32_t* out_count);
 
+/* Search of the meso level only (two-level indexes): the top-level PDXearch<F32> over the L1
+ * centroids that PDXTreeIndex::Append runs to pick a cluster (index.hpp:596-607). Queries are
+ * already normalised + rotated (host buffers); nprobe applies to the meso clusters. Output ids
+ * are L1 cluster ids, ascending by distance. */
+pdxg_status pdxg_search_meso(pdxg_index* index, const float* queries_transformed, uint32_t num_queries,
+                             uint32_t k, uint32_t nprobe, uint32_t* out_ids, float* out_distances,
+                             uint32_t* out_counts);
+
 const char* pdxg_last_error(void);
 
 /* Library / device description: "gfx950 <name> <CUs> CUs <LDS> B LDS". */

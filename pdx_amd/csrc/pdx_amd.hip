@@ -742,6 +742,50 @@ pdxg_status pdxg_preprocess(pdxg_index* ix, const float* queries, uint32_t nq, u
     });
 }
 
+pdxg_status pdxg_search_meso(pdxg_index* ix, const float* queries, uint32_t nq, uint32_t k, uint32_t nprobe,
+                             uint32_t* out_ids, float* out_dists, uint32_t* out_counts) {
+    return guarded([&] {
+        require(ix && queries && out_ids && out_counts, "null argument");
+        require(ix->tree(), "meso search needs a two-level index");
+        require(k >= 1 && k <= kMaxK, "k must be in [1, 2048]", PDXG_ERR_UNSUPPORTED);
+        DeviceGuard g(ix->device);
+        std::lock_guard<std::mutex> lk(ix->mu);
+        if (nq == 0) return;
+        pdxg_workspace* ws = host_workspace(ix, nq, ix->ws ? ix->ws->max_k : 1u, ix->ws ? ix->ws->max_nprobe : 1u);
+        ix->h_queries.ensure((size_t)nq * ix->d);
+        ix->h_queries.upload(queries, (size_t)nq * ix->d);
+        launch_preprocess(ix, *ws, ix->h_queries.p, nq, true, nullptr);
+        const uint32_t n0 = ix->l0.nc;
+        const uint32_t nv = (nprobe == 0 || nprobe > n0) ? n0 : nprobe;  // searcher.hpp:614-616
+        std::vector<uint32_t> hk(nq, k), hnv(nq, nv);
+        ws->k0.upload(hk.data(), nq);
+        ws->nv0.upload(hnv.data(), nq);
+        launch_probe(ix, ix->l0, ws->qrot.p, nq, ws->nv0.p, ws->order.p, ws->order_stride, ws->probe.p, nullptr);
+        const size_t no = (size_t)nq * k;
+        ix->h_ids.ensure(no);
+        ix->h_dists.ensure(no);
+        ix->h_counts.ensure(nq);
+        ScanArgs s{};
+        s.L = ix->l0.view(1.0f, 1.0f);
+        s.ratios = ix->ratios.p;
+        s.qf = ws->qrot.p;
+        s.order = ws->order.p;
+        s.order_stride = ws->order_stride;
+        s.nvisit = ws->nv0.p;
+        s.kq = ws->k0.p;
+        s.out_ids = ix->h_ids.p;
+        s.out_dists = ix->h_dists.p;
+        s.out_counts = ix->h_counts.p;
+        s.out_stride = k;
+        s.nq = nq;
+        launch_scan<false, false>(ix, ix->l0, s, k, nullptr);
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out_counts, ix->h_counts.p, (size_t)nq * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(out_ids, ix->h_ids.p, no * 4, hipMemcpyDeviceToHost));
+        if (out_dists) HIP_CHECK(hipMemcpy(out_dists, ix->h_dists.p, no * 4, hipMemcpyDeviceToHost));
+    });
+}
+
 pdxg_status pdxg_filtered_search(pdxg_index* ix, const float* query, uint32_t transformed, uint32_t k,
                                  uint32_t nprobe, const uint8_t* selection, uint32_t* out_ids, float* out_dists,
                                  uint32_t* out_idists, uint32_t* out_count) {

@@ -28,7 +28,8 @@ __device__ __forceinline__ uint64_t make_key(float dist, uint32_t id) {
 __device__ __forceinline__ float key_dist(uint64_t k) { return u2f(static_cast<uint32_t>(k >> 32)); }
 __device__ __forceinline__ uint32_t key_id(uint64_t k) { return static_cast<uint32_t>(k); }
 
-// Rotate-right within each 16-lane DPP row; lane l receives lane (l+N)%16.
+// DPP row rotate-right by N within each 16-lane row: lane l receives lane (l - N) mod 16, so
+// rotating by 16 - M reads lane l + M.
 template <int N>
 __device__ __forceinline__ float row_ror(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x120 + N, 0xf, 0xf, false));
@@ -38,13 +39,13 @@ __device__ __forceinline__ uint32_t row_ror_u(uint32_t v) {
     return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x120 + N, 0xf, 0xf, false));
 }
 
-// AVX-512 reduction tree over the 16 lanes of a DPP row; lane 0 of the row holds the result.
-// (row_ror 8: x_i = s_i + s_{i+8}; ror 4: r_i = x_i + x_{i+4}; ror 1: r0+r1 / r2+r3; ror 2: total.)
+// AVX-512 reduction tree over the 16 lanes of a DPP row; lane 0 of the row holds the result:
+// x_i = s_i + s_{i+8}; r_i = x_i + x_{i+4}; (r_0 + r_1) + (r_2 + r_3).
 __device__ __forceinline__ float tree16(float acc) {
-    acc = acc + row_ror<8>(acc);
-    acc = acc + row_ror<4>(acc);
-    acc = acc + row_ror<1>(acc);
-    acc = acc + row_ror<2>(acc);
+    acc = acc + row_ror<8>(acc);   // + lane l+8
+    acc = acc + row_ror<12>(acc);  // + lane l+4
+    acc = acc + row_ror<15>(acc);  // + lane l+1
+    acc = acc + row_ror<14>(acc);  // + lane l+2
     return acc;
 }
 __device__ __forceinline__ uint32_t sum16_u(uint32_t acc) {
