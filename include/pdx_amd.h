@@ -119,6 +119,14 @@ pdxg_status pdxg_workspace_destroy(pdxg_workspace* ws);
 size_t pdxg_workspace_bytes(const pdxg_index* index, uint32_t max_queries, uint32_t max_k,
                             uint32_t max_nprobe);
 
+/* Per-stage device timing of searches run with this workspace: when enabled, every
+ * pdxg_search_device call records HIP events on its stream around its stages;
+ * pdxg_workspace_stage_times waits for the last call and writes the elapsed milliseconds of
+ * [0] preprocessing (normalise + rotate + quantise), [1] probe (centroid distances + ordering),
+ * [2] meso-level scan (two-level indexes), [3] data-level scan. Not in the reference. */
+pdxg_status pdxg_workspace_enable_timing(pdxg_workspace* ws, int enable);
+pdxg_status pdxg_workspace_stage_times(pdxg_workspace* ws, float* out_ms4);
+
 /* Asynchronous batched search on `stream` (a hipStream_t, NULL = default stream) with device
  * buffers. The workspace must not be shared by calls that overlap in time. */
 pdxg_status pdxg_search_device(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,

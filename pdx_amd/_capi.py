@@ -64,7 +64,7 @@ class SearchArgs(ctypes.Structure):
 EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_index_device_bytes",
            "pdxg_search", "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_workspace_bytes",
            "pdxg_search_device", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
-           "pdxg_device_info"]
+           "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times"]
 
 _lib = None
 
@@ -93,9 +93,13 @@ def lib():
     L.pdxg_preprocess.argtypes = [vp, f32p, u32, u32, f32p, u8p]
     L.pdxg_filtered_search.argtypes = [vp, f32p, u32, u32, u32, u8p, u32p, f32p, u32p, u32p]
     L.pdxg_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    L.pdxg_search_meso.argtypes = [vp, f32p, u32, u32, u32, u32p, f32p, u32p]
+    L.pdxg_workspace_enable_timing.argtypes = [vp, ctypes.c_int]
+    L.pdxg_workspace_stage_times.argtypes = [vp, f32p]
     for name in ("pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_search",
                  "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device", "pdxg_preprocess",
-                 "pdxg_filtered_search", "pdxg_device_info"):
+                 "pdxg_filtered_search", "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing",
+                 "pdxg_workspace_stage_times"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -249,6 +253,15 @@ class Workspace:
             self.close()
         except Exception:
             pass
+
+    def enable_timing(self, on: bool = True) -> None:
+        check(lib().pdxg_workspace_enable_timing(self._h, int(on)))
+
+    def stage_times(self):
+        """Milliseconds of [preprocess, probe, meso scan, data scan] of the last search."""
+        out = np.zeros(4, np.float32)
+        check(lib().pdxg_workspace_stage_times(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
 
     def search(self, queries_ptr: int, nq: int, k_ptr: int, nprobe_ptr: int, k_max: int, nprobe_max: int,
                out_stride: int, out_ids: int, out_dists: int, out_idists: int, out_counts: int,

@@ -87,6 +87,27 @@ class _Backend:
             out[s:s + chunk] = (at @ bt).cpu().numpy()
         return out
 
+    def cluster_sums(self, x: np.ndarray, a: np.ndarray, k: int) -> np.ndarray:
+        """Per-cluster float64 sums of the rows of x grouped by assignment a."""
+        if self.torch is None:
+            order = np.argsort(a, kind="stable")
+            xs = x[order].astype(np.float64)
+            starts = np.searchsorted(a[order], np.arange(k))
+            sums = np.zeros((k, x.shape[1]), np.float64)
+            nz = np.nonzero(np.bincount(a, minlength=k))[0]
+            if len(nz):
+                red = np.add.reduceat(xs, starts[nz], axis=0)
+                sums[nz] = red
+            return sums
+        t = self.torch
+        out = t.zeros((k, x.shape[1]), dtype=t.float64, device=self.dev)
+        chunk = 1 << 16
+        for s in range(0, x.shape[0], chunk):
+            xt = t.from_numpy(np.ascontiguousarray(x[s:s + chunk])).to(self.dev).double()
+            at = t.from_numpy(a[s:s + chunk]).to(self.dev)
+            out.index_add_(0, at, xt)
+        return out.cpu().numpy()
+
     def assign(self, x: np.ndarray, c: np.ndarray, chunk: int = 1 << 16) -> np.ndarray:
         """Nearest centroid by L2 (ties: lowest index)."""
         cn = np.einsum("ij,ij->i", c, c).astype(np.float32)
@@ -121,8 +142,7 @@ def kmeans(x: np.ndarray, k: int, iters: int, seed: int, sampling_fraction: floa
     for _ in range(iters):
         a = backend.assign(train, cent)
         counts = np.bincount(a, minlength=k).astype(np.float32)
-        sums = np.zeros((k, d), np.float64)
-        np.add.at(sums, a, train)
+        sums = backend.cluster_sums(train, a, k)
         empty = counts == 0
         cent = (sums / np.maximum(counts, 1)[:, None]).astype(np.float32)
         if empty.any():

@@ -55,7 +55,6 @@ pdxg_status guarded(F&& f) {
 // On-chip top-k buffers are sized by the largest k of a call; beyond this the GPU path reports
 // PDXG_ERR_UNSUPPORTED rather than silently degrading.
 constexpr uint32_t kMaxK = 2048;
-constexpr uint32_t kScanBlock = 256;
 constexpr uint32_t kMaxProbeSort = 8192;  // k_probe_select sorts next_pow2(nc) keys in LDS
 
 // adsampling.hpp:147-157
@@ -261,6 +260,15 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
 
 struct pdxg_workspace {
     int device = 0;
+    bool timing = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    ~pdxg_workspace() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    void mark(int i, hipStream_t st) {
+        if (timing) (void)hipEventRecord(ev[i], st);
+    }
     uint32_t max_queries = 0, max_k = 0, max_nprobe = 0;
     uint32_t k0_max = 0, order_stride = 0;
     DevBuf<float> qnorm, qrot, probe;
@@ -353,9 +361,9 @@ void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
     }
 }
 
-template <bool U8, bool FILT>
-void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
-    LdsPlan p = plan_scan_lds(L, kmax, kScanBlock);
+template <bool U8, bool FILT, uint32_t BLOCK>
+void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
+    LdsPlan p = plan_scan_lds(L, kmax, BLOCK);
     require(p.bytes <= (uint32_t)ix->max_lds,
             "cluster capacity / k too large for the on-chip scan buffers (" + std::to_string(p.bytes) + " B LDS)",
             PDXG_ERR_UNSUPPORTED);
@@ -372,13 +380,23 @@ void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax
     a.o_ck = p.o_ck;
     a.o_cp = p.o_cp;
     a.o_red = p.o_red;
-    auto kern = k_scan<U8, FILT, kScanBlock>;
+    auto kern = k_scan<U8, FILT, BLOCK>;
     if (p.bytes > 65536)
         HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)p.bytes));
     if (a.nq == 0) return;
-    hipLaunchKernelGGL(kern, dim3(a.nq), dim3(kScanBlock), p.bytes, st, a);
+    hipLaunchKernelGGL(kern, dim3(a.nq), dim3(BLOCK), p.bytes, st, a);
     HIP_CHECK(hipGetLastError());
+}
+
+// One workgroup per query. Batches too small to give every CU a few workgroups get 1024-thread
+// workgroups (more loads in flight per query); larger batches 256-thread ones (more queries per CU).
+constexpr uint32_t kSmallBatch = 128;
+
+template <bool U8, bool FILT>
+void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
+    if (a.nq < kSmallBatch) launch_scan_block<U8, FILT, 1024>(ix, L, a, kmax, st);
+    else launch_scan_block<U8, FILT, 256>(ix, L, a, kmax, st);
 }
 
 void launch_probe(const pdxg_index* ix, const Level& L, const float* qrot, uint32_t nq, const uint32_t* nvisit,
@@ -386,8 +404,18 @@ void launch_probe(const pdxg_index* ix, const Level& L, const float* qrot, uint3
     require(L.has_centroids, "level has no centroids on device", PDXG_ERR_UNSUPPORTED);
     require(L.nc <= kMaxProbeSort, "too many centroids for the on-chip probe sort", PDXG_ERR_UNSUPPORTED);
     if (nq == 0) return;
-    dim3 g1(nq, (L.nc + kProbeCPB - 1) / kProbeCPB);
-    hipLaunchKernelGGL(k_probe_dist, g1, dim3(256), ix->d * 4, st, qrot, L.centroids.p, probe, ix->d, L.nc, nq);
+    // queries per workgroup: 8 while their LDS copy fits in 64 KB, fewer for small batches / large d
+    const uint32_t ngy = (L.nc + kProbeCPB - 1) / kProbeCPB;
+    if (nq >= 64 && 8u * ix->d * 4 <= 65536) {
+        dim3 g1((nq + 7) / 8, ngy);
+        hipLaunchKernelGGL(k_probe_dist<8>, g1, dim3(256), 8 * ix->d * 4, st, qrot, L.centroids.p, probe, ix->d, L.nc, nq);
+    } else if (nq >= 16 && 4u * ix->d * 4 <= 65536) {
+        dim3 g1((nq + 3) / 4, ngy);
+        hipLaunchKernelGGL(k_probe_dist<4>, g1, dim3(256), 4 * ix->d * 4, st, qrot, L.centroids.p, probe, ix->d, L.nc, nq);
+    } else {
+        dim3 g1(nq, ngy);
+        hipLaunchKernelGGL(k_probe_dist<1>, g1, dim3(256), ix->d * 4, st, qrot, L.centroids.p, probe, ix->d, L.nc, nq);
+    }
     HIP_CHECK(hipGetLastError());
     uint32_t n2 = 1;
     while (n2 < L.nc) n2 <<= 1;
@@ -430,6 +458,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
     if (!a.nprobe) require(a.nprobe_all == np_cap || np_cap == 0 || a.nprobe_all <= np_cap, "nprobe_all > nprobe_max");
     if (nq == 0) return;
 
+    ws.mark(0, st);
     launch_preprocess(ix, ws, a.queries, nq, a.queries_transformed != 0, st);
 
     SetupArgs sa{};
@@ -460,8 +489,11 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
     base.out_bytes = reinterpret_cast<unsigned long long*>(a.out_bytes);
     base.nq = nq;
 
+    ws.mark(1, st);
     if (!ix->tree()) {
         launch_probe(ix, ix->l1, qrot, nq, ws.nv1.p, ws.order.p, ws.order_stride, ws.probe.p, st);
+        ws.mark(2, st);
+        ws.mark(3, st);
         ScanArgs s = base;
         s.L = ix->l1.view(ix->qscale_sq, ix->inv_qscale_sq);
         s.qf = qrot;
@@ -478,11 +510,13 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s.out_stride = a.out_stride;
         if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s, a.k_max, st);
         else launch_scan<false, false>(ix, ix->l1, s, a.k_max, st);
+        ws.mark(4, st);
     } else {
         require(ws.k0_max <= kMaxK,
                 "nprobe above " + std::to_string(kMaxK) + " (or 0 with that many clusters) is not supported on the "
                 "GPU path", PDXG_ERR_UNSUPPORTED);
         launch_probe(ix, ix->l0, qrot, nq, ws.nv0.p, ws.order.p, ws.order_stride, ws.probe.p, st);
+        ws.mark(2, st);
         ScanArgs s0 = base;
         s0.L = ix->l0.view(1.0f, 1.0f);
         s0.qf = qrot;
@@ -496,6 +530,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s0.out_counts = ws.cnt0.p;
         s0.out_stride = ws.k0_max;
         launch_scan<false, false>(ix, ix->l0, s0, ws.k0_max, st);
+        ws.mark(3, st);
         ScanArgs s1 = base;
         s1.L = ix->l1.view(ix->qscale_sq, ix->inv_qscale_sq);
         s1.qf = qrot;
@@ -512,6 +547,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s1.out_stride = a.out_stride;
         if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s1, a.k_max, st);
         else launch_scan<false, false>(ix, ix->l1, s1, a.k_max, st);
+        ws.mark(4, st);
     }
 }
 
@@ -656,6 +692,27 @@ pdxg_status pdxg_workspace_destroy(pdxg_workspace* ws) {
         if (!ws) return;
         DeviceGuard g(ws->device);
         delete ws;
+    });
+}
+
+pdxg_status pdxg_workspace_enable_timing(pdxg_workspace* ws, int enable) {
+    return guarded([&] {
+        require(ws != nullptr, "null argument");
+        DeviceGuard g(ws->device);
+        if (enable)
+            for (auto& e : ws->ev)
+                if (!e) HIP_CHECK(hipEventCreate(&e));
+        ws->timing = enable != 0;
+    });
+}
+
+pdxg_status pdxg_workspace_stage_times(pdxg_workspace* ws, float* out) {
+    return guarded([&] {
+        require(ws && out, "null argument");
+        require(ws->timing, "timing is not enabled on this workspace");
+        DeviceGuard g(ws->device);
+        HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+        for (int i = 0; i < 4; ++i) HIP_CHECK(hipEventElapsedTime(&out[i], ws->ev[i], ws->ev[i + 1]));
     });
 }
 

@@ -195,16 +195,21 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ x, con
 // ------------------------------------------------------------------------------------------
 // IVF probe
 // ------------------------------------------------------------------------------------------
-// dist[q][c] = l2sq16(q, centroid c). One workgroup per (query, 64 centroids); each 16-lane row
-// computes one centroid at a time.
+// dist[q][c] = l2sq16(q, centroid c). One workgroup per (QT queries, 64 centroids): the QT queries
+// sit in LDS and each 16-lane row streams one centroid at a time, reusing every centroid load for
+// all QT queries (QT accumulators per lane, each in the 16-lane order of l2sq16).
 constexpr uint32_t kProbeCPB = 64;
+template <uint32_t QT>
 __global__ __launch_bounds__(256) void k_probe_dist(const float* __restrict__ qf, const float* __restrict__ cent,
                                                     float* __restrict__ out, uint32_t d, uint32_t nc, uint32_t nq) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
-    float* qs = reinterpret_cast<float*>(lds_raw);
-    const uint32_t q = blockIdx.x;
-    if (q >= nq) return;
-    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) qs[j] = qf[(size_t)q * d + j];
+    float* qs = reinterpret_cast<float*>(lds_raw);  // [QT][d]
+    const uint32_t q0 = blockIdx.x * QT;
+    const uint32_t nqt = min(QT, nq - q0);
+    for (uint32_t t = threadIdx.x; t < QT * d; t += blockDim.x) {
+        uint32_t qq = t / d, j = t % d;
+        qs[t] = qq < nqt ? qf[(size_t)(q0 + qq) * d + j] : 0.0f;
+    }
     __syncthreads();
     const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16, G = blockDim.x / 16;
     const uint32_t c0 = blockIdx.y * kProbeCPB;
@@ -212,13 +217,22 @@ __global__ __launch_bounds__(256) void k_probe_dist(const float* __restrict__ qf
         uint32_t c = c0 + cc;
         if (c >= nc) break;  // uniform across the 16-lane row
         const float* cr = cent + (size_t)c * d;
-        float acc = 0.0f;
+        float acc[QT];
+#pragma unroll
+        for (uint32_t r = 0; r < QT; ++r) acc[r] = 0.0f;
         for (uint32_t j = l; j < d; j += 16) {
-            float df = qs[j] - cr[j];
-            acc = __builtin_fmaf(df, df, acc);
+            const float x = cr[j];
+#pragma unroll
+            for (uint32_t r = 0; r < QT; ++r) {
+                float df = qs[r * d + j] - x;
+                acc[r] = __builtin_fmaf(df, df, acc[r]);
+            }
         }
-        acc = tree16(acc);
-        if (l == 0) out[(size_t)q * nc + c] = acc;
+#pragma unroll
+        for (uint32_t r = 0; r < QT; ++r) {
+            float s = tree16(acc[r]);
+            if (l == 0 && r < nqt) out[(size_t)(q0 + r) * nc + c] = s;
+        }
     }
 }
 
