@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--plain-queries", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work per baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-batch", type=int, default=1024)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--sigma-per-coord", action="store_true", help="apply sigma per coordinate (see mixture())")
     ap.add_argument("--target-recall", type=float, default=0.95)
@@ -240,6 +243,10 @@ def main():
     log(f"plain per-query: {plain_qps:.0f} QPS p50 {np.percentile(lat1, 50):.3f} ms p99 {np.percentile(lat1, 99):.3f} ms; "
         f"batch-64 p50 {np.percentile(lat64, 50):.3f} ms")
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_leg(_capi, g, q, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(idx, q, args)
@@ -275,10 +282,40 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "serving": serving,
         "cpu_baseline": cpu,
         "build_s": round(build_s, 1),
     }
     print(json.dumps(line), flush=True)
+
+
+def serve_leg(_capi, g, q, args):
+    """Continuous batching: saturated QPS, then Poisson arrivals at fractions of it (p50/p99)."""
+    sv = _capi.Server(g, max_batch=args.serve_batch, max_k=args.k, max_nprobe=args.nprobe,
+                      max_slots=args.serve_slots)
+    rng = np.random.default_rng(7)
+    n_sat = min(len(q), 8 * args.serve_batch)
+    lat, el = sv.replay(q[:n_sat], args.k, args.nprobe, np.zeros(n_sat))
+    sv.replay(q[:n_sat], args.k, args.nprobe, np.zeros(n_sat))  # second pass: warm
+    lat, el = sv.replay(q[:n_sat], args.k, args.nprobe, np.zeros(n_sat))
+    sat = n_sat / el
+    out = {"saturated_qps": round(sat, 1), "slots": sv.stats()["slots"], "max_batch": args.serve_batch,
+           "scratch_bytes": sv.stats()["scratch_bytes"], "load": {}}
+    for frac in (0.1, 0.5, 0.9):
+        rate = frac * sat
+        n = int(min(60000, max(2000, rate * 0.4)))
+        idx = rng.integers(0, len(q), n)
+        arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        lat, el = sv.replay(q[idx], args.k, args.nprobe, arr)
+        out["load"][f"{int(frac * 100)}%"] = {"offered_qps": round(rate, 1), "served_qps": round(n / el, 1),
+                                              "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                                              "p99_ms": round(float(np.percentile(lat, 99)), 4)}
+        log(f"serving at {int(frac * 100)}% of {sat:.0f} QPS: p50 {np.percentile(lat, 50):.3f} ms "
+            f"p99 {np.percentile(lat, 99):.3f} ms")
+    st = sv.stats()
+    out["mean_batch"] = round(st["completed"] / max(1, st["batches"]), 1)
+    sv.close()
+    return out
 
 
 def cpu_baseline(idx, q, args):

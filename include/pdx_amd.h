@@ -155,6 +155,51 @@ pdxg_status pdxg_search_meso(pdxg_index* index, const float* queries_transformed
                              uint32_t k, uint32_t nprobe, uint32_t* out_ids, float* out_distances,
                              uint32_t* out_counts);
 
+/* ---------------------------------------------------------------------------------------------
+ * Serving: continuous batching over a resident index (not in the reference, which serves one
+ * query per call on one thread). Requests (one query each, own k and nprobe) are queued by
+ * pdxg_server_submit; a dispatcher thread packs the waiting requests into a batch whenever one of
+ * the server's slots is free and launches it on that slot's stream, so batches overlap on the
+ * device; a completion thread returns results in completion order. The number of slots follows
+ * from the scratch budget: each slot owns one workspace of max_batch x max_k x max_nprobe
+ * (pdxg_workspace_bytes), so admission is bounded by scratch memory. Per-query results do not
+ * depend on what a query is batched with.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct pdxg_server pdxg_server;
+
+typedef struct {
+    uint32_t max_batch;          /* queries per launch */
+    uint32_t max_k;              /* largest accepted k */
+    uint32_t max_nprobe;         /* largest accepted nprobe; 0 = any, including "all" */
+    uint32_t max_slots;          /* upper bound on batches in flight (streams) */
+    size_t scratch_budget_bytes; /* device scratch for all slots; 0 = no limit beyond max_slots */
+} pdxg_server_config;
+
+typedef struct {
+    uint64_t submitted, completed, batches;
+    uint32_t slots;              /* slots admitted by the scratch budget */
+    size_t scratch_bytes;        /* device scratch held by the slots */
+} pdxg_server_stats;
+
+pdxg_status pdxg_server_create(pdxg_index* index, const pdxg_server_config* config, pdxg_server** out);
+/* Stops the threads after the queued requests complete. */
+pdxg_status pdxg_server_destroy(pdxg_server* server);
+/* Enqueue one query (host buffer, d floats, copied). tag identifies the result. */
+pdxg_status pdxg_server_submit(pdxg_server* server, const float* query, uint32_t k, uint32_t nprobe,
+                               uint64_t tag);
+/* Collect up to max_results completed requests, waiting up to timeout_ms for the first one.
+ * Row r of out_ids / out_distances (stride out_stride >= max_k) belongs to out_tags[r];
+ * out_latency_ms (optional) is submit-to-completion time. */
+pdxg_status pdxg_server_poll(pdxg_server* server, uint32_t max_results, int timeout_ms, uint64_t* out_tags,
+                             uint32_t out_stride, uint32_t* out_ids, float* out_distances, uint32_t* out_counts,
+                             float* out_latency_ms, uint32_t* num_results);
+pdxg_status pdxg_server_get_stats(pdxg_server* server, pdxg_server_stats* out);
+/* Load driver: submit query i (row i of queries, k[i], nprobe[i]) at arrival_s[i] seconds after the
+ * call starts (ascending), collect every result, and write its submit-to-completion latency in ms.
+ * Returns the wall time from first arrival to last completion in *elapsed_s. */
+pdxg_status pdxg_server_replay(pdxg_server* server, const float* queries, const uint32_t* k, const uint32_t* nprobe,
+                               const double* arrival_s, uint32_t n, float* out_latency_ms, double* elapsed_s);
+
 const char* pdxg_last_error(void);
 
 /* Library / device description: "gfx950 <name> <CUs> CUs <LDS> B LDS". */

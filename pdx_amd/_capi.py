@@ -51,6 +51,16 @@ class IndexDesc(ctypes.Structure):
                 ("l1", LevelDesc), ("l0", LevelDesc)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_uint32), ("max_k", ctypes.c_uint32), ("max_nprobe", ctypes.c_uint32),
+                ("max_slots", ctypes.c_uint32), ("scratch_budget_bytes", ctypes.c_size_t)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("slots", ctypes.c_uint32), ("scratch_bytes", ctypes.c_size_t)]
+
+
 class SearchArgs(ctypes.Structure):
     _fields_ = [("num_queries", ctypes.c_uint32), ("queries", ctypes.c_void_p),
                 ("queries_transformed", ctypes.c_uint32), ("k", ctypes.c_void_p), ("nprobe", ctypes.c_void_p),
@@ -64,7 +74,9 @@ class SearchArgs(ctypes.Structure):
 EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_index_device_bytes",
            "pdxg_search", "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_workspace_bytes",
            "pdxg_search_device", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
-           "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times"]
+           "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times",
+           "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
+           "pdxg_server_get_stats", "pdxg_server_replay"]
 
 _lib = None
 
@@ -96,10 +108,18 @@ def lib():
     L.pdxg_search_meso.argtypes = [vp, f32p, u32, u32, u32, u32p, f32p, u32p]
     L.pdxg_workspace_enable_timing.argtypes = [vp, ctypes.c_int]
     L.pdxg_workspace_stage_times.argtypes = [vp, f32p]
+    u64p, dblp = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)
+    L.pdxg_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    L.pdxg_server_destroy.argtypes = [vp]
+    L.pdxg_server_submit.argtypes = [vp, f32p, u32, u32, ctypes.c_uint64]
+    L.pdxg_server_poll.argtypes = [vp, u32, ctypes.c_int, u64p, u32, u32p, f32p, u32p, f32p, u32p]
+    L.pdxg_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.pdxg_server_replay.argtypes = [vp, f32p, u32p, u32p, dblp, u32, f32p, dblp]
     for name in ("pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_search",
                  "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device", "pdxg_preprocess",
                  "pdxg_filtered_search", "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing",
-                 "pdxg_workspace_stage_times"):
+                 "pdxg_workspace_stage_times", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit",
+                 "pdxg_server_poll", "pdxg_server_get_stats", "pdxg_server_replay"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -272,3 +292,67 @@ class Workspace:
                        nprobe_all, k_max, nprobe_max, out_stride, out_ids or None, out_dists or None,
                        out_idists or None, out_counts or None, out_bytes or None)
         check(lib().pdxg_search_device(self.index.handle, self._h, ctypes.byref(a), ctypes.c_void_p(stream or 0)))
+
+
+class Server:
+    """Continuous-batching server over a resident index (pdxg_server_*)."""
+
+    def __init__(self, index: GpuIndex, max_batch: int = 1024, max_k: int = 100, max_nprobe: int = 0,
+                 max_slots: int = 4, scratch_budget_bytes: int = 0):
+        self.index = index
+        self.max_k = max_k
+        cfg = ServerConfig(max_batch, max_k, max_nprobe, max_slots, scratch_budget_bytes)
+        h = ctypes.c_void_p()
+        check(lib().pdxg_server_create(index.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().pdxg_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, query: np.ndarray, k: int, nprobe: int, tag: int) -> None:
+        q = np.ascontiguousarray(query, np.float32)
+        check(lib().pdxg_server_submit(self._h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), k, nprobe, tag))
+
+    def poll(self, max_results: int = 1024, timeout_ms: int = 100):
+        """Returns a list of (tag, ids, dists, latency_ms)."""
+        K = self.max_k
+        tags = np.zeros(max_results, np.uint64)
+        ids = np.zeros((max_results, K), np.uint32)
+        d = np.zeros((max_results, K), np.float32)
+        cnt = np.zeros(max_results, np.uint32)
+        lat = np.zeros(max_results, np.float32)
+        n = ctypes.c_uint32()
+        p = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))
+        check(lib().pdxg_server_poll(self._h, max_results, timeout_ms, p(tags, ctypes.c_uint64), K,
+                                     p(ids, ctypes.c_uint32), p(d, ctypes.c_float), p(cnt, ctypes.c_uint32),
+                                     p(lat, ctypes.c_float), ctypes.byref(n)))
+        return [(int(tags[i]), ids[i, :cnt[i]].copy(), d[i, :cnt[i]].copy(), float(lat[i])) for i in range(n.value)]
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(lib().pdxg_server_get_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def replay(self, queries: np.ndarray, k, nprobe, arrival_s: np.ndarray):
+        """Timed load: query i is submitted at arrival_s[i]; returns (latency_ms per query, elapsed_s)."""
+        q = np.ascontiguousarray(queries, np.float32)
+        n = q.shape[0]
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (n,)))
+        npb = np.ascontiguousarray(np.broadcast_to(np.asarray(nprobe, np.uint32), (n,)))
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        lat = np.zeros(n, np.float32)
+        el = ctypes.c_double()
+        check(lib().pdxg_server_replay(self._h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       kk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                       npb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                       arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                       lat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(el)))
+        return lat, el.value

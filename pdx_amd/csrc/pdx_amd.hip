@@ -905,3 +905,5 @@ pdxg_status pdxg_filtered_search(pdxg_index* ix, const float* query, uint32_t tr
 }
 
 }  // extern "C"
+
+#include "pdx_server.inc"

@@ -1,0 +1,84 @@
+"""Continuous-batching server: every result is the same alone, in a mixed batch and under timed
+load, and equals the oracle's; admission follows the scratch budget."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleIndex
+from pdx_amd import pdxfile as pf
+from tests.pdx_testdata import index_file
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    from pdx_amd import _capi
+
+    path, x, q = index_file("pdx_tree_u8", 4000, 768, 40)
+    rng = np.random.default_rng(5)
+    qs = np.concatenate([q, q + 0.01 * rng.standard_normal(q.shape).astype(np.float32)])
+    k = rng.choice([10, 20, 100], len(qs)).astype(np.uint32)
+    nprobe = rng.choice([4, 16, 0], len(qs)).astype(np.uint32)
+    g = _capi.GpuIndex(pf.load(path))
+    return _capi, path, g, qs, k, nprobe
+
+
+def _collect(sv, n):
+    out = {}
+    while len(out) < n:
+        for tag, ids, d, lat in sv.poll(256, 2000):
+            out[tag] = (ids, d, lat)
+    return out
+
+
+def test_alone_equals_mixed_equals_oracle(setup):
+    _capi, path, g, qs, k, nprobe = setup
+    sv = _capi.Server(g, max_batch=16, max_k=100, max_nprobe=0, max_slots=3)
+    alone = {}
+    for i in range(len(qs)):
+        sv.submit(qs[i], int(k[i]), int(nprobe[i]), i)
+        alone.update(_collect(sv, 1))
+    for i in range(len(qs)):
+        sv.submit(qs[i], int(k[i]), int(nprobe[i]), 1000 + i)
+    mixed = _collect(sv, len(qs))
+    st = sv.stats()
+    assert st["completed"] == 2 * len(qs) and st["slots"] == 3
+    o = OracleIndex(path)
+    oi, od, _, oc, _ = o.search(qs, k, nprobe)
+    for i in range(len(qs)):
+        a_ids, a_d, _ = alone[i]
+        m_ids, m_d, _ = mixed[1000 + i]
+        np.testing.assert_array_equal(a_ids, m_ids)
+        np.testing.assert_array_equal(a_d.view(np.uint32), m_d.view(np.uint32))
+        np.testing.assert_array_equal(a_ids, oi[i, :oc[i]])
+        np.testing.assert_array_equal(a_d.view(np.uint32), od[i, :oc[i]].view(np.uint32))
+    sv.close()
+
+
+def test_replay_returns_every_result_with_latency(setup):
+    _capi, path, g, qs, k, nprobe = setup
+    sv = _capi.Server(g, max_batch=8, max_k=100, max_nprobe=0, max_slots=4)
+    rng = np.random.default_rng(6)
+    arrivals = np.cumsum(rng.exponential(1e-4, len(qs)))
+    lat, elapsed = sv.replay(qs, k, nprobe, arrivals)
+    assert np.all(lat > 0) and elapsed > 0
+    assert sv.stats()["completed"] == len(qs)
+    sv.close()
+
+
+def test_scratch_budget_bounds_the_slots(setup):
+    _capi, path, g, qs, k, nprobe = setup
+    one = _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8)
+    per_slot = one.stats()["scratch_bytes"] // one.stats()["slots"]
+    one.close()
+    two = _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8, scratch_budget_bytes=2 * per_slot + 1)
+    assert two.stats()["slots"] == 2
+    two.close()
+    with pytest.raises(_capi.PdxgError):
+        _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8, scratch_budget_bytes=per_slot // 2)
+    s = _capi.Server(g, max_batch=4, max_k=20, max_nprobe=16, max_slots=1)
+    with pytest.raises(_capi.PdxgError):
+        s.submit(qs[0], 50, 4, 0)       # k above max_k
+    with pytest.raises(_capi.PdxgError):
+        s.submit(qs[0], 10, 0, 0)       # nprobe "all" above max_nprobe
+    s.close()
