@@ -1,15 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the PDX search path on one MI355X.
 
-Workload (default): IVF float32 with ADSampling. N = 1,000,000 unit-norm vectors of d = 768 from a
-1,024-component Gaussian mixture (centres ~ N(0, I) normalised, sigma = 0.15, rows renormalised),
-1,000 IVF lists, 10,000 queries from the mixture, k = 10, nprobe = 32, L2 metric.
+Default workload (--config c2): IVF float32 with ADSampling. N = 1,000,000 unit-norm vectors of
+d = 768 from a 1,024-component Gaussian mixture (centres ~ N(0, I) normalised, noise norm 0.15,
+rows renormalised), 1,000 IVF lists, 10,000 queries from the mixture, k = 10, nprobe = 32, L2.
 
-A step is one batched search of --batch queries (default 1,024) through the device-buffer C-ABI
-(pdxg_search_device) with queries, index and outputs resident in HBM. The line also carries the
-plain per-query GPU path (one query per call), batch-64 latency percentiles, recall@10 against an
-FP64 brute-force ground truth, the roofline of the dominant kernel (the data-level scan) and the CPU
-baseline (the oracle, i.e. the reference search semantics restated in C++, on the host's cores).
+--config c3: two-level IVF SQ8 (IndexPDXIVFTreeSQ8): N = 5,000,000, d = 1,024, 4,096-component
+mixture (centres ~ N(0, I), per-coordinate sigma 0.2), normalised as the Python API does by
+default, 17,889 lists / 133 meso clusters, k = 20, batches of 4,096 queries with per-query nprobe
+drawn uniformly from [8, 128]; results are checked bit-exact against the oracle in the run.
+
+A step is one batched search of --batch queries through the device-buffer C-ABI
+(pdxg_search_device) with queries, index and outputs resident in HBM. The JSON line also carries
+the plain per-query GPU path (one query per call), batch-64 latency percentiles, recall@k against
+an FP64 brute-force ground truth, the continuous-batching server (saturated QPS, p50/p99 under
+Poisson load), the roofline of the dominant scan kernel and the CPU baseline (the oracle, i.e.
+the reference search semantics restated in C++, on the host's cores).
 """
 from __future__ import annotations
 
@@ -17,7 +23,6 @@ import argparse
 import json
 import os
 import sys
-import tempfile
 import time
 
 import numpy as np
@@ -27,86 +32,103 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+CONFIGS = {
+    "c2": dict(n=1_000_000, d=768, centers=1024, sigma=0.15, unit=True, kind="pdx_f32", nlist=1000, n_meso=0,
+               normalize=False, nq=10_000, batch=1024, k=10, nprobe=32, nprobe_range=None, gt=1000),
+    "c3": dict(n=5_000_000, d=1024, centers=4096, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
+               normalize=True, nq=8192, batch=4096, k=20, nprobe=0, nprobe_range=(8, 128), gt=512),
+}
+
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def mixture(n, d, n_centers, sigma, seed, n_queries, per_coord=False):
-    """Unit-norm Gaussian mixture rows and queries (centres normalised, rows renormalised).
+def mixture(n, d, n_centers, sigma, seed, n_queries, unit=True, per_coord=False):
+    """Gaussian mixture rows and queries.
 
-    sigma is the noise scale relative to the unit-norm centres: per-coordinate std sigma / sqrt(d),
-    so the noise vector has norm ~sigma. per_coord=True applies sigma per coordinate instead (noise
-    norm sigma * sqrt(d), which swamps unit-norm centres at d = 768)."""
+    unit=True: centres normalised, rows renormalised, and sigma is the noise scale relative to the
+    unit centres (per-coordinate std sigma / sqrt(d), noise norm ~sigma) unless per_coord.
+    unit=False: centres ~ N(0, I) and per-coordinate std sigma."""
     rng = np.random.default_rng(seed)
     c = rng.standard_normal((n_centers, d), dtype=np.float32)
-    c /= np.linalg.norm(c, axis=1, keepdims=True)
-    sigma = sigma if per_coord else sigma / np.sqrt(d)
+    if unit:
+        c /= np.linalg.norm(c, axis=1, keepdims=True)
+        if not per_coord:
+            sigma = sigma / np.sqrt(d)
     x = np.empty((n, d), np.float32)
     chunk = 1 << 17
     for s in range(0, n, chunk):
         m = min(chunk, n - s)
-        lab = rng.integers(0, n_centers, m)
-        blk = c[lab] + np.float32(sigma) * rng.standard_normal((m, d), dtype=np.float32)
-        blk /= np.linalg.norm(blk, axis=1, keepdims=True)
+        blk = c[rng.integers(0, n_centers, m)] + np.float32(sigma) * rng.standard_normal((m, d), dtype=np.float32)
+        if unit:
+            blk /= np.linalg.norm(blk, axis=1, keepdims=True)
         x[s:s + m] = blk
-    lab = rng.integers(0, n_centers, n_queries)
-    q = c[lab] + np.float32(sigma) * rng.standard_normal((n_queries, d), dtype=np.float32)
-    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q = c[rng.integers(0, n_centers, n_queries)] + np.float32(sigma) * rng.standard_normal((n_queries, d), dtype=np.float32)
+    if unit:
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
     return x, q.astype(np.float32)
 
 
-def ground_truth(x, q, k, torch, dev):
-    """Exact top-k ids by L2 in float64 on the GPU."""
-    xt = torch.from_numpy(x).to(dev, torch.float64)
-    xn = (xt * xt).sum(1)
-    out = []
-    for s in range(0, q.shape[0], 256):
-        qt = torch.from_numpy(q[s:s + 256]).to(dev, torch.float64)
-        dd = xn[None, :] - 2.0 * (qt @ xt.T) + (qt * qt).sum(1)[:, None]
-        out.append(torch.topk(dd, k, dim=1, largest=False).indices.cpu().numpy())
-    del xt
+def ground_truth(x, q, k, normalize, torch, dev, chunk=1 << 20):
+    """Exact top-k ids by L2 in float64 on the GPU (rows and queries normalised if the index is)."""
+    qt = torch.from_numpy(q).to(dev, torch.float64)
+    if normalize:
+        qt = qt / torch.linalg.vector_norm(qt, dim=1, keepdim=True)
+    qn = (qt * qt).sum(1)
+    best_d = torch.full((q.shape[0], k), float("inf"), dtype=torch.float64, device=dev)
+    best_i = torch.zeros((q.shape[0], k), dtype=torch.int64, device=dev)
+    for s in range(0, x.shape[0], chunk):
+        xt = torch.from_numpy(x[s:s + chunk]).to(dev, torch.float64)
+        if normalize:
+            xt = xt / torch.linalg.vector_norm(xt, dim=1, keepdim=True)
+        dd = (xt * xt).sum(1)[None, :] - 2.0 * (qt @ xt.T) + qn[:, None]
+        vd, vi = torch.topk(dd, min(k, dd.shape[1]), dim=1, largest=False)
+        cd = torch.cat([best_d, vd], 1)
+        ci = torch.cat([best_i, vi + s], 1)
+        o = torch.argsort(cd, dim=1)[:, :k]
+        best_d, best_i = torch.gather(cd, 1, o), torch.gather(ci, 1, o)
+    out = best_i.cpu().numpy()
     torch.cuda.empty_cache()
-    return np.concatenate(out)
+    return out
 
 
 def recall(ids, counts, gt, k):
-    r = [len(set(ids[i, :counts[i]].tolist()) & set(gt[i, :k].tolist())) / k for i in range(len(gt))]
-    return float(np.mean(r))
+    return float(np.mean([len(set(ids[i, :counts[i]].tolist()) & set(gt[i, :k].tolist())) / k for i in range(len(gt))]))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--d", type=int, default=768)
-    ap.add_argument("--centers", type=int, default=1024)
-    ap.add_argument("--sigma", type=float, default=0.15)
-    ap.add_argument("--nlist", type=int, default=1000)
-    ap.add_argument("--nq", type=int, default=10_000)
-    ap.add_argument("--batch", type=int, default=1024)
-    ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--nprobe", type=int, default=32)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    for key in ("n", "d", "centers", "nlist", "nq", "batch", "k", "nprobe"):
+        ap.add_argument("--" + key, type=int, default=None)
+    ap.add_argument("--sigma", type=float, default=None)
     ap.add_argument("--metric", default="l2sq", choices=["l2sq", "ip"])
-    ap.add_argument("--gt-queries", type=int, default=1000)
     ap.add_argument("--plain-queries", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work per baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-batch", type=int, default=1024)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--sigma-per-coord", action="store_true", help="apply sigma per coordinate (see mixture())")
+    ap.add_argument("--sigma-per-coord", action="store_true", help="unit-norm mixtures: sigma per coordinate")
     ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--parity-queries", type=int, default=256)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
+    cfg = dict(CONFIGS[args.config])
+    for key in ("n", "d", "centers", "nlist", "nq", "batch", "k", "nprobe", "sigma"):
+        if getattr(args, key) is not None:
+            cfg[key] = getattr(args, key)
+    if args.metric == "ip":
+        cfg["normalize"] = True
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the index would be "
-              "replicated per GPU with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (each GPU would hold a "
+              "replica of the index and serve its own queries)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -116,20 +138,24 @@ def main():
     dev = torch.device("cuda", 0)
     log("device:", _capi.device_info(0))
     t0 = time.time()
-    x, q = mixture(args.n, args.d, args.centers, args.sigma, args.seed, args.nq, args.sigma_per_coord)
+    x, q = mixture(cfg["n"], cfg["d"], cfg["centers"], cfg["sigma"], args.seed, cfg["nq"], cfg["unit"],
+                   args.sigma_per_coord)
     log(f"data {x.shape} in {time.time() - t0:.1f}s")
     t0 = time.time()
-    idx = builder.build_index(x, "pdx_f32", num_clusters=args.nlist, normalize=args.metric == "ip",
-                              distance_metric=args.metric, seed=42, device=0)
+    idx = builder.build_index(x, cfg["kind"], num_clusters=cfg["nlist"], num_meso_clusters=cfg["n_meso"],
+                              normalize=cfg["normalize"], distance_metric=args.metric, seed=42, device=0)
     build_s = time.time() - t0
     sizes = np.array([c.n_live for c in idx.l1.clusters])
-    log(f"index built in {build_s:.1f}s; cluster sizes min/mean/max {sizes.min()}/{sizes.mean():.0f}/{sizes.max()}")
+    nlist = idx.l1.nc
+    log(f"index {cfg['kind']} built in {build_s:.1f}s; {nlist} lists"
+        + (f", {idx.l0.nc} meso" if idx.l0 is not None else "")
+        + f"; sizes min/mean/max {sizes.min()}/{sizes.mean():.0f}/{sizes.max()}")
     g = _capi.GpuIndex(idx)
     log(f"index resident: {g.device_bytes / 2**30:.2f} GiB")
 
-    gtq = min(args.gt_queries, args.nq)
+    gtq = min(cfg["gt"], cfg["nq"])
     t0 = time.time()
-    gt = ground_truth(x, q[:gtq], args.k, torch, dev)
+    gt = ground_truth(x, q[:gtq], cfg["k"], cfg["normalize"], torch, dev)
     log(f"fp64 ground truth for {gtq} queries in {time.time() - t0:.1f}s")
 
     if args.copy_calib_mib:
@@ -140,78 +166,106 @@ def main():
         log(f"copy calibration: {args.copy_calib_mib} MiB device-to-device copy")
         del a, bcopy
 
-    B, K = args.batch, args.k
+    B, K, NQ = cfg["batch"], cfg["k"], cfg["nq"]
+    rng = np.random.default_rng(args.seed + 1)
+    if cfg["nprobe_range"]:
+        lo, hi = cfg["nprobe_range"]
+        nprobe_q = rng.integers(lo, hi + 1, NQ).astype(np.uint32)
+    else:
+        nprobe_q = np.full(NQ, cfg["nprobe"], np.uint32)
     qd = torch.from_numpy(q).to(dev)
     out_ids = torch.zeros((B, K), dtype=torch.int32, device=dev)
     out_d = torch.zeros((B, K), dtype=torch.float32, device=dev)
+    out_i = torch.zeros((B, K), dtype=torch.int32, device=dev)
     out_c = torch.zeros(B, dtype=torch.int32, device=dev)
-    out_b = torch.zeros(B, dtype=torch.int64, device=dev)
-    ws = _capi.Workspace(g, B, K, 0)
-    ws.enable_timing(True)
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
-    nbatches = args.nq // B
+    sp = torch.cuda.current_stream().cuda_stream
+    state = {}
+    tree = idx.l0 is not None
 
-    def run(qs, nq, want_bytes=False, w=ws):
-        w.search(qs.data_ptr(), nq, 0, 0, K, args.nprobe, K, out_ids.data_ptr(), out_d.data_ptr(), 0,
-                 out_c.data_ptr(), out_b.data_ptr() if want_bytes else 0, stream=sp, k_all=K,
-                 nprobe_all=args.nprobe)
+    def workspace(nmax, np_max):
+        # two-level searches size the meso-level result by the largest nprobe; flat ones take any
+        return _capi.Workspace(g, nmax, K, np_max if tree else 0)
 
-    # operating point: the configured nprobe, doubled until recall@k on the ground-truth subset
+    def run(s, m, w=None):
+        w = w or state["ws"]
+        npd = state["np_dev"]
+        w.search(qd[s:s + m].data_ptr(), m, 0, npd[s:s + m].data_ptr(), K, state["np_max"], K,
+                 out_ids.data_ptr(), out_d.data_ptr(), out_i.data_ptr(), out_c.data_ptr(), 0, stream=sp, k_all=K)
+
+    def set_nprobe(arr):
+        state["np_dev"] = torch.from_numpy(arr.astype(np.int32)).to(dev)
+        state["np_max"] = int(arr.max())
+        state["ws"] = workspace(B, state["np_max"])
+        state["ws"].enable_timing(True)
+
+    # operating point: with a fixed nprobe, double it until recall@k on the ground-truth subset
     # reaches the target (the metric is QPS at recall@10 >= 0.95)
-    def recall_at(npb):
+    def recall_now():
         ids = np.zeros((gtq, K), np.uint32)
         cnt = np.zeros(gtq, np.uint32)
         for s in range(0, gtq, B):
             m = min(B, gtq - s)
-            ws.search(qd[s:s + m].data_ptr(), m, 0, 0, K, npb, K, out_ids.data_ptr(), out_d.data_ptr(), 0,
-                      out_c.data_ptr(), 0, stream=sp, k_all=K, nprobe_all=npb)
+            run(s, m)
             torch.cuda.synchronize()
             ids[s:s + m] = out_ids[:m].cpu().numpy().view(np.uint32)
             cnt[s:s + m] = out_c[:m].cpu().numpy()
         return recall(ids, cnt, gt, K)
 
     sweep = {}
-    npb = args.nprobe
     while True:
-        sweep[npb] = recall_at(npb)
-        log(f"recall@{K} = {sweep[npb]:.4f} at nprobe {npb}")
-        if sweep[npb] >= args.target_recall or npb >= args.nlist:
+        set_nprobe(nprobe_q)
+        key = str(int(nprobe_q[0])) if not cfg["nprobe_range"] else "U[%d,%d]" % cfg["nprobe_range"]
+        sweep[key] = recall_now()
+        log(f"recall@{K} = {sweep[key]:.4f} at nprobe {key}")
+        if cfg["nprobe_range"] or sweep[key] >= args.target_recall or nprobe_q[0] >= nlist:
             break
-        npb = min(args.nlist, npb * 2)
-    args.nprobe = npb
-    rec = sweep[npb]
+        nprobe_q[:] = min(nlist, int(nprobe_q[0]) * 2)
+    rec = sweep[key]
 
-    # algorithmic bytes of the data-level scan per batch (probe bytes subtracted)
-    scan_bytes = []
-    for b in range(min(nbatches, 4)):
-        run(qd[b * B:(b + 1) * B], B, want_bytes=True)
-        torch.cuda.synchronize()
-        scan_bytes.append(int(out_b.sum().item()) - B * args.nlist * args.d * 4)
+    # autotune the scan workgroup size per level on one batch ("waves per cluster")
+    def tune(w, m):
+        npd = state["np_dev"]
+        return w.autotune(qd[:m].data_ptr(), m, 0, npd[:m].data_ptr(), K, state["np_max"], K, out_ids.data_ptr(),
+                          out_d.data_ptr(), out_i.data_ptr(), out_c.data_ptr(), stream=sp, k_all=K)
 
-    # timed leg: K steps of one batch each, inputs resident in HBM
+    plan = tune(state["ws"], B)
+    log(f"autotuned scan block sizes (meso, data): {plan}")
+
+    parity = None
+    if args.parity_queries:
+        parity = check_parity(idx, q, nprobe_q, K, min(args.parity_queries, B), run, out_ids, out_d, out_i,
+                              out_c, torch)
+
+    # timed leg: `steps` batches, inputs resident in HBM, one batch in flight at a time
+    nb = max(1, NQ // B)
     for i in range(args.warmup):
-        b = i % nbatches
-        run(qd[b * B:(b + 1) * B], B)
+        run((i % nb) * B, B)
     torch.cuda.synchronize()
     stage = np.zeros(4)
+    sbytes = np.zeros(2)
     t0 = time.perf_counter()
+    ws = state["ws"]
     for i in range(args.steps):
-        b = i % nbatches
-        run(qd[b * B:(b + 1) * B], B)
-        stage += ws.stage_times()  # waits for this batch: one batch in flight at a time
+        run((i % nb) * B, B)
+        stage += ws.stage_times()
+        sbytes += ws.stage_bytes()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     qps = B * args.steps / elapsed
     stage /= args.steps
-    log(f"batched: {ms_per_step:.3f} ms/step, {qps:.0f} QPS; stage ms preprocess/probe/-/scan = {stage.round(4)}")
-
-    bytes_per_launch = float(np.mean(scan_bytes))
-    scan_ms = float(stage[3])
-    achieved = bytes_per_launch / (scan_ms * 1e-3) / 1e9
-    workload = (f"IVF F32 ADSampling, N={args.n}, d={args.d}, {args.nlist} lists, {args.metric}, k={K}, "
-                f"nprobe={args.nprobe}, batch={B}")
+    sbytes /= args.steps
+    log(f"batched: {ms_per_step:.3f} ms/step, {qps:.0f} QPS; stage ms preprocess/probe/meso/scan = {stage.round(4)}; "
+        f"scan bytes meso/data = {sbytes[0]:.3e}/{sbytes[1]:.3e}")
+    # dominant kernel: the longer of the two scans (the meso scan only exists for two-level indexes)
+    dom = 1 if stage[3] >= stage[2] else 0
+    dom_ms = float(stage[3 if dom else 2])
+    achieved = sbytes[dom] / (dom_ms * 1e-3) / 1e9
+    u8 = cfg["kind"].endswith("u8")
+    kernel = (("k_scan<U8>" if u8 else "k_scan<F32>") + " (data-level pruned PDX scan)") if dom else \
+        "k_scan<F32> (meso-level pruned PDX scan)"
+    workload = (f"{cfg['kind']}, N={cfg['n']}, d={cfg['d']}, {nlist} lists, {args.metric}, k={K}, nprobe={key}, "
+                f"batch={B}")
     traffic = None  # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile of this workload
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
@@ -219,21 +273,24 @@ def main():
     except (OSError, ValueError):
         pass
 
-    # batch-64 latency and plain per-query path (one query per call)
-    ws64 = _capi.Workspace(g, 64, K, args.nprobe)
+    # batch-64 latency and the plain per-query path (one query per call)
+    ws64 = workspace(64, state["np_max"])
+    plan64 = tune(ws64, 64)
     lat64 = []
-    for i in range(min(64, args.nq // 64)):
+    for i in range(min(64, NQ // 64)):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        run(qd[i * 64:(i + 1) * 64], 64, w=ws64)
+        run(i * 64, 64, w=ws64)
         torch.cuda.synchronize()
         lat64.append(time.perf_counter() - t)
-    ws1 = _capi.Workspace(g, 1, K, args.nprobe)
+    ws1 = workspace(1, state["np_max"])
+    plan1 = tune(ws1, 1)
+    log(f"autotuned plans: batch 64 {plan64}, one query {plan1}")
     lat1 = []
     for i in range(args.plain_queries + 5):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        run(qd[i:i + 1], 1, w=ws1)
+        run(i, 1, w=ws1)
         torch.cuda.synchronize()
         if i >= 5:
             lat1.append(time.perf_counter() - t)
@@ -243,13 +300,8 @@ def main():
     log(f"plain per-query: {plain_qps:.0f} QPS p50 {np.percentile(lat1, 50):.3f} ms p99 {np.percentile(lat1, 99):.3f} ms; "
         f"batch-64 p50 {np.percentile(lat64, 50):.3f} ms")
 
-    serving = None
-    if not args.no_serve:
-        serving = serve_leg(_capi, g, q, args)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(idx, q, args)
+    serving = None if args.no_serve else serve_leg(_capi, g, q, nprobe_q, cfg, args, plan)
+    cpu = None if args.no_cpu else cpu_baseline(idx, q, nprobe_q, K, args)
 
     line = {
         "metric": "QPS at recall@10>=0.95 (batched GPU search; plain per-query GPU path and p50/p99 reported beside it)",
@@ -262,26 +314,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded unit-norm Gaussian mixture, noise norm sigma; index built in-process)",
-        "config": {"workload": workload,
-                   "global_batch": B, "parallelism": "single GPU"},
-        "recall_at_10": round(rec, 4),
-        "nprobe_used": args.nprobe,
-        "recall_sweep": {str(k): round(v, 4) for k, v in sweep.items()},
+        "dtype": "u8 (int32 accumulation)" if u8 else "f32",
+        "data": "synthetic (seeded Gaussian mixture; index built in-process)",
+        "config": {"workload": workload, "global_batch": B, "parallelism": "single GPU",
+                   "scan_block": {"batch": list(plan), "batch64": list(plan64), "one_query": list(plan1)}},
+        f"recall_at_{K}": round(rec, 4),
+        "recall_sweep": sweep,
+        "parity_vs_oracle": parity,
         "plain_per_query_qps": round(plain_qps, 1),
         "batched_over_plain": round(qps / plain_qps, 2),
         "latency_ms": {"plain_p50": round(float(np.percentile(lat1, 50)), 4),
                        "plain_p99": round(float(np.percentile(lat1, 99)), 4),
                        "batch64_p50": round(float(np.percentile(lat64, 50)), 4),
                        "batch64_p99": round(float(np.percentile(lat64, 99)), 4),
-                       "batch1024_mean": round(ms_per_step, 4)},
+                       f"batch{B}_mean": round(ms_per_step, 4)},
         "stage_ms": {"preprocess": round(float(stage[0]), 4), "probe": round(float(stage[1]), 4),
-                     "scan": round(scan_ms, 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_scan<F32> (data-level pruned PDX scan)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+                     "meso_scan": round(float(stage[2]), 4), "scan": round(float(stage[3]), 4)},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": int(sbytes[dom]),
+                     "meso_scan_bytes": int(sbytes[0]), "scan_bytes": int(sbytes[1])},
         "serving": serving,
         "cpu_baseline": cpu,
         "build_s": round(build_s, 1),
@@ -289,24 +341,48 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def serve_leg(_capi, g, q, args):
+def check_parity(idx, q, nprobe_q, K, n, run, out_ids, out_d, out_i, out_c, torch):
+    """GPU results of the first n queries vs the oracle: ids, int32 and float distances bit-exact."""
+    from oracle.oracle import OracleIndex
+
+    t0 = time.time()
+    o = OracleIndex(data=idx)
+    oi, od, oid, oc, _ = o.search(q[:n], K, nprobe_q[:n], threads=min(16, os.cpu_count() or 1))
+    del o
+    run(0, n)
+    torch.cuda.synchronize()
+    gi = out_ids[:n].cpu().numpy().view(np.uint32)
+    gd = out_d[:n].cpu().numpy()
+    gid = out_i[:n].cpu().numpy().view(np.uint32)
+    gc = out_c[:n].cpu().numpy().view(np.uint32)
+    u8 = idx.is_u8
+    ok_c = bool(np.array_equal(gc, oc))
+    ok_i = ok_c and all(np.array_equal(gi[r, :oc[r]], oi[r, :oc[r]]) for r in range(n))
+    ok_d = (ok_c and all(np.array_equal(gid[r, :oc[r]], oid[r, :oc[r]]) for r in range(n))) if u8 else None
+    ok_f = ok_c and all(np.array_equal(gd[r, :oc[r]].view(np.uint32), od[r, :oc[r]].view(np.uint32)) for r in range(n))
+    log(f"parity vs oracle on {n} queries: ids {ok_i} int32 distances {ok_d} float distances {ok_f} "
+        f"({time.time() - t0:.1f}s)")
+    return {"queries": n, "ids_equal": ok_i, "int32_distances_equal": ok_d, "float_distances_equal": ok_f}
+
+
+def serve_leg(_capi, g, q, nprobe_q, cfg, args, plan):
     """Continuous batching: saturated QPS, then Poisson arrivals at fractions of it (p50/p99)."""
-    sv = _capi.Server(g, max_batch=args.serve_batch, max_k=args.k, max_nprobe=args.nprobe,
-                      max_slots=args.serve_slots)
+    K = cfg["k"]
+    sv = _capi.Server(g, max_batch=cfg["batch"], max_k=K, max_nprobe=int(nprobe_q.max()), max_slots=args.serve_slots,
+                      scan_block=plan)
     rng = np.random.default_rng(7)
-    n_sat = min(len(q), 8 * args.serve_batch)
-    lat, el = sv.replay(q[:n_sat], args.k, args.nprobe, np.zeros(n_sat))
-    sv.replay(q[:n_sat], args.k, args.nprobe, np.zeros(n_sat))  # second pass: warm
-    lat, el = sv.replay(q[:n_sat], args.k, args.nprobe, np.zeros(n_sat))
+    n_sat = min(len(q), 8 * cfg["batch"])
+    for _ in range(2):  # second pass: warm
+        lat, el = sv.replay(q[:n_sat], K, nprobe_q[:n_sat], np.zeros(n_sat))
     sat = n_sat / el
-    out = {"saturated_qps": round(sat, 1), "slots": sv.stats()["slots"], "max_batch": args.serve_batch,
+    out = {"saturated_qps": round(sat, 1), "slots": sv.stats()["slots"], "max_batch": cfg["batch"],
            "scratch_bytes": sv.stats()["scratch_bytes"], "load": {}}
     for frac in (0.1, 0.5, 0.9):
         rate = frac * sat
         n = int(min(60000, max(2000, rate * 0.4)))
-        idx = rng.integers(0, len(q), n)
+        sel = rng.integers(0, len(q), n)
         arr = np.cumsum(rng.exponential(1.0 / rate, n))
-        lat, el = sv.replay(q[idx], args.k, args.nprobe, arr)
+        lat, el = sv.replay(q[sel], K, nprobe_q[sel], arr)
         out["load"][f"{int(frac * 100)}%"] = {"offered_qps": round(rate, 1), "served_qps": round(n / el, 1),
                                               "p50_ms": round(float(np.percentile(lat, 50)), 4),
                                               "p99_ms": round(float(np.percentile(lat, 99)), 4)}
@@ -318,34 +394,29 @@ def serve_leg(_capi, g, q, args):
     return out
 
 
-def cpu_baseline(idx, q, args):
+def cpu_baseline(idx, q, nprobe_q, K, args):
     """The oracle (reference semantics, C++) on the host cores, over a bounded query sample."""
     from oracle import oracle as orc
-    from pdx_amd import pdxfile as pf
 
-    path = os.path.join(tempfile.gettempdir(), f"pdx_bench_{os.getpid()}.pdx")
-    pf.save(idx, path)
-    o = orc.OracleIndex(path)
-    os.remove(path)
-    qr = o.preprocess(q[:5000])
-    # single thread: calibrate the sample to ~cpu_seconds
+    o = orc.OracleIndex(data=idx)
+    nq = min(len(q), 5000)
+    qr = o.preprocess(q[:nq])
+    npb = nprobe_q[:nq]
     t = time.perf_counter()
-    o.search(qr[:20], args.k, args.nprobe, transformed=True, threads=1)
+    o.search(qr[:20], K, npb[:20], transformed=True, threads=1)
     per_q = (time.perf_counter() - t) / 20
-    n1 = int(max(20, min(len(qr), args.cpu_seconds / max(per_q, 1e-6))))
+    n1 = int(max(20, min(nq, args.cpu_seconds / max(per_q, 1e-6))))
     t = time.perf_counter()
-    o.search(qr[:n1], args.k, args.nprobe, transformed=True, threads=1)
+    o.search(qr[:n1], K, npb[:n1], transformed=True, threads=1)
     qps1 = n1 / (time.perf_counter() - t)
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1))
-    nq = int(min(len(qr), max(cores * 20, qps1 * cores * args.cpu_seconds / 4)))
     t = time.perf_counter()
-    o.search(qr[:nq], args.k, args.nprobe, transformed=True, threads=cores)
+    o.search(qr, K, npb, transformed=True, threads=cores)
     qpsn = nq / (time.perf_counter() - t)
     log(f"cpu oracle ({orc.isa()}): 1 thread {qps1:.0f} QPS over {n1} queries; {cores} threads {qpsn:.0f} QPS over {nq}")
     return {"value": round(qpsn, 1), "unit": "queries/s", "cores": cores, "kind": "port",
             "single_thread_qps": round(qps1, 1), "isa": orc.isa(),
-            "sample": f"{nq} pre-rotated queries on {cores} threads ({n1} on 1 thread), same index, k={args.k}, "
-                      f"nprobe={args.nprobe}"}
+            "sample": f"{nq} pre-rotated queries on {cores} threads ({n1} on 1 thread), same index and nprobe"}
 
 
 if __name__ == "__main__":

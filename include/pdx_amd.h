@@ -126,6 +126,19 @@ size_t pdxg_workspace_bytes(const pdxg_index* index, uint32_t max_queries, uint3
  * [2] meso-level scan (two-level indexes), [3] data-level scan. Not in the reference. */
 pdxg_status pdxg_workspace_enable_timing(pdxg_workspace* ws, int enable);
 pdxg_status pdxg_workspace_stage_times(pdxg_workspace* ws, float* out_ms4);
+/* Algorithmic bytes read by [0] the meso-level scan and [1] the data-level scan of the last timed
+ * search (summed over its queries; centroid probe bytes excluded). */
+pdxg_status pdxg_workspace_stage_bytes(pdxg_workspace* ws, uint64_t* out2);
+
+/* Scan workgroup size per level ("waves per cluster": 64 .. 1024 threads per query; 0 = the
+ * built-in heuristic). pdxg_workspace_autotune runs the given search (device buffers, as for
+ * pdxg_search_device) with every size, level by level, and keeps the fastest; it writes the
+ * outputs of the search as a side effect. Results never depend on the sizes. Not in the
+ * reference. */
+pdxg_status pdxg_workspace_set_plan(pdxg_workspace* ws, uint32_t block_meso, uint32_t block_data);
+pdxg_status pdxg_workspace_get_plan(pdxg_workspace* ws, uint32_t* out2);
+pdxg_status pdxg_workspace_autotune(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,
+                                    void* stream);
 
 /* Asynchronous batched search on `stream` (a hipStream_t, NULL = default stream) with device
  * buffers. The workspace must not be shared by calls that overlap in time. */
@@ -173,6 +186,7 @@ typedef struct {
     uint32_t max_nprobe;         /* largest accepted nprobe; 0 = any, including "all" */
     uint32_t max_slots;          /* upper bound on batches in flight (streams) */
     size_t scratch_budget_bytes; /* device scratch for all slots; 0 = no limit beyond max_slots */
+    uint32_t scan_block[2];      /* scan workgroup sizes (meso, data), as pdxg_workspace_set_plan */
 } pdxg_server_config;
 
 typedef struct {

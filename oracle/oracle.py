@@ -74,6 +74,9 @@ def lib():
     L.orc_filtered_search.restype = ctypes.c_int
     L.orc_filtered_search.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int, ctypes.c_uint32,
                                       ctypes.c_uint32, _u8p, _u32p, _f32p, _u32p, _u32p]
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                             _f32p, ctypes.c_uint32, ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_void_p, _f32p]
     L.orc_cluster_offsets.argtypes = [ctypes.c_void_p, _u64p]
     L.orc_cluster_ids.restype = ctypes.c_uint32
     L.orc_cluster_ids.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _u32p]
@@ -126,11 +129,43 @@ def quantize(x: np.ndarray, base: float, scale: float) -> np.ndarray:
     return out
 
 
-class OracleIndex:
-    """A PDX index file loaded by the oracle."""
+class _OrcCluster(ctypes.Structure):
+    _fields_ = [("n_live", ctypes.c_uint32), ("n_used", ctypes.c_uint32), ("cap", ctypes.c_uint32),
+                ("data", ctypes.c_void_p), ("ids", ctypes.c_void_p), ("tombstones", ctypes.c_void_p),
+                ("n_tombstones", ctypes.c_uint32)]
 
-    def __init__(self, path: str):
-        h = lib().orc_load(path.encode())
+
+def _clusters(level, keep):
+    arr = (_OrcCluster * level.nc)()
+    for i, c in enumerate(level.clusters):
+        tomb = np.array(sorted(c.tombstones), np.uint32)
+        data = np.ascontiguousarray(c.data)
+        ids = np.ascontiguousarray(c.ids, np.uint32)
+        keep += [tomb, data, ids]
+        arr[i] = _OrcCluster(c.n_live, c.n_used, c.cap, data.ctypes.data, ids.ctypes.data, tomb.ctypes.data, len(tomb))
+    keep.append(arr)
+    return arr
+
+
+class OracleIndex:
+    """A PDX index loaded by the oracle from a file (or built from an in-memory IndexData)."""
+
+    def __init__(self, path: str = None, data=None):
+        if data is not None:
+            keep = []
+            L, M = data.l1, data.l0
+            rot = np.ascontiguousarray(data.matrix, np.float32)
+            c1 = np.ascontiguousarray(L.centroids, np.float32) if L.centroids is not None else None
+            c0 = np.ascontiguousarray(M.centroids, np.float32) if M is not None else None
+            keep += [rot, c1, c0]
+            h = lib().orc_create(data.type, L.d, int(L.normalized), L.base, L.scale, _p(rot, _f32p), L.nc,
+                                 ctypes.cast(_clusters(L, keep), ctypes.c_void_p),
+                                 _p(c1, _f32p) if c1 is not None else None,
+                                 M.nc if M is not None else 0,
+                                 ctypes.cast(_clusters(M, keep), ctypes.c_void_p) if M is not None else None,
+                                 _p(c0, _f32p) if c0 is not None else None)
+        else:
+            h = lib().orc_load(path.encode())
         if not h:
             raise RuntimeError(lib().orc_last_error().decode())
         self._h = ctypes.c_void_p(h)

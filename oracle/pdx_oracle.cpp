@@ -834,6 +834,68 @@ int orc_filtered_search(void* h, const float* query, int transformed, uint32_t k
     }
 }
 
+// In-memory construction (same content as a file, without the file): one cluster description per
+// cluster, strided PDX buffers with stride = max_capacity (cluster.hpp:17-105).
+struct orc_cluster {
+    uint32_t n_live, n_used, cap;
+    const void* data;
+    const uint32_t* ids;
+    const uint32_t* tombstones;
+    uint32_t n_tombstones;
+};
+
+static void fill_level(orc::Level& L, uint32_t nc, const orc_cluster* cs, const float* cent) {
+    L.nc = nc;
+    L.clusters.resize(nc);
+    for (uint32_t c = 0; c < nc; ++c) {
+        const orc_cluster& x = cs[c];
+        orc::Cluster& cl = L.clusters[c];
+        cl.n_live = x.n_live;
+        cl.n_used = x.n_used;
+        cl.cap = x.cap;
+        const size_t bytes = (size_t)x.cap * L.d * L.elem();
+        cl.data.assign(static_cast<const uint8_t*>(x.data), static_cast<const uint8_t*>(x.data) + bytes);
+        cl.ids.assign(x.cap, 0);
+        std::memcpy(cl.ids.data(), x.ids, sizeof(uint32_t) * x.n_used);
+        for (uint32_t t = 0; t < x.n_tombstones; ++t) cl.tombstones.insert(x.tombstones[t]);
+    }
+    if (cent) L.centroids.assign(cent, cent + (size_t)nc * L.d);
+    L.compute_offsets();
+}
+
+void* orc_create(uint32_t type, uint32_t d, uint32_t normalized, float base, float scale, const float* rotation,
+                 uint32_t nc1, const orc_cluster* c1, const float* cent1, uint32_t nc0, const orc_cluster* c0,
+                 const float* cent0) {
+    try {
+        auto idx = std::make_unique<orc::Index>();
+        idx->type = (uint8_t)type;
+        idx->mrows = idx->mcols = d;
+        idx->matrix.assign(rotation, rotation + (size_t)d * d);
+        orc::Split s = orc::dimension_split(d);
+        const bool u8 = type == 1 || type == 3;
+        orc::Level& L = idx->l1;
+        L.d = d; L.v = s.v; L.h = s.h; L.u8 = u8; L.normalized = normalized != 0;
+        fill_level(L, nc1, c1, cent1);
+        if (u8) {
+            L.base = base;
+            L.scale = scale;
+            L.scale_sq = scale * scale;
+            L.inv_scale_sq = 1.0f / L.scale_sq;
+        }
+        if (type >= 2) {
+            orc::Level& M = idx->l0;
+            M.d = d; M.v = s.v; M.h = s.h; M.u8 = false; M.normalized = L.normalized;
+            fill_level(M, nc0, c0, cent0);
+        }
+        idx->ratios.resize(d);
+        for (uint32_t i = 0; i < d; ++i) idx->ratios[i] = orc::ratio_at(i, d);
+        return idx.release();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
 // Level-1 cluster offsets (cluster_offsets of ivf_wrapper.hpp:76-87), nc entries + total.
 void orc_cluster_offsets(void* h, uint64_t* out) {
     auto* idx = static_cast<orc::Index*>(h);

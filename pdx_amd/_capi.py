@@ -53,7 +53,8 @@ class IndexDesc(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("max_k", ctypes.c_uint32), ("max_nprobe", ctypes.c_uint32),
-                ("max_slots", ctypes.c_uint32), ("scratch_budget_bytes", ctypes.c_size_t)]
+                ("max_slots", ctypes.c_uint32), ("scratch_budget_bytes", ctypes.c_size_t),
+                ("scan_block", ctypes.c_uint32 * 2)]
 
 
 class ServerStats(ctypes.Structure):
@@ -75,7 +76,8 @@ EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster
            "pdxg_search", "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_workspace_bytes",
            "pdxg_search_device", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
            "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times",
-           "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
+           "pdxg_workspace_stage_bytes",
+           "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
            "pdxg_server_get_stats", "pdxg_server_replay"]
 
 _lib = None
@@ -108,6 +110,10 @@ def lib():
     L.pdxg_search_meso.argtypes = [vp, f32p, u32, u32, u32, u32p, f32p, u32p]
     L.pdxg_workspace_enable_timing.argtypes = [vp, ctypes.c_int]
     L.pdxg_workspace_stage_times.argtypes = [vp, f32p]
+    L.pdxg_workspace_stage_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.pdxg_workspace_set_plan.argtypes = [vp, u32, u32]
+    L.pdxg_workspace_get_plan.argtypes = [vp, u32p]
+    L.pdxg_workspace_autotune.argtypes = [vp, vp, ctypes.POINTER(SearchArgs), vp]
     u64p, dblp = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)
     L.pdxg_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     L.pdxg_server_destroy.argtypes = [vp]
@@ -118,7 +124,8 @@ def lib():
     for name in ("pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_search",
                  "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device", "pdxg_preprocess",
                  "pdxg_filtered_search", "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing",
-                 "pdxg_workspace_stage_times", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit",
+                 "pdxg_workspace_stage_times", "pdxg_workspace_stage_bytes", "pdxg_workspace_set_plan",
+                 "pdxg_workspace_get_plan", "pdxg_workspace_autotune", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit",
                  "pdxg_server_poll", "pdxg_server_get_stats", "pdxg_server_replay"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
@@ -283,6 +290,30 @@ class Workspace:
         check(lib().pdxg_workspace_stage_times(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
 
+    def set_plan(self, block_meso: int = 0, block_data: int = 0) -> None:
+        check(lib().pdxg_workspace_set_plan(self._h, block_meso, block_data))
+
+    def plan(self):
+        out = np.zeros(2, np.uint32)
+        check(lib().pdxg_workspace_get_plan(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return int(out[0]), int(out[1])
+
+    def autotune(self, queries_ptr: int, nq: int, k_ptr: int, nprobe_ptr: int, k_max: int, nprobe_max: int,
+                 out_stride: int, out_ids: int, out_dists: int, out_idists: int, out_counts: int, stream: int = 0,
+                 k_all: int = 0, nprobe_all: int = 0):
+        """Pick the fastest scan block size per level on this search; returns (meso, data)."""
+        a = SearchArgs(nq, queries_ptr or None, 0, k_ptr or None, nprobe_ptr or None, k_all, nprobe_all, k_max,
+                       nprobe_max, out_stride, out_ids or None, out_dists or None, out_idists or None,
+                       out_counts or None, None)
+        check(lib().pdxg_workspace_autotune(self.index.handle, self._h, ctypes.byref(a), ctypes.c_void_p(stream or 0)))
+        return self.plan()
+
+    def stage_bytes(self):
+        """Algorithmic bytes of [meso scan, data scan] of the last search."""
+        out = np.zeros(2, np.uint64)
+        check(lib().pdxg_workspace_stage_bytes(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return out
+
     def search(self, queries_ptr: int, nq: int, k_ptr: int, nprobe_ptr: int, k_max: int, nprobe_max: int,
                out_stride: int, out_ids: int, out_dists: int, out_idists: int, out_counts: int,
                out_bytes: int = 0, transformed: bool = False, stream: int = 0, k_all: int = 0,
@@ -298,10 +329,11 @@ class Server:
     """Continuous-batching server over a resident index (pdxg_server_*)."""
 
     def __init__(self, index: GpuIndex, max_batch: int = 1024, max_k: int = 100, max_nprobe: int = 0,
-                 max_slots: int = 4, scratch_budget_bytes: int = 0):
+                 max_slots: int = 4, scratch_budget_bytes: int = 0, scan_block=(0, 0)):
         self.index = index
         self.max_k = max_k
-        cfg = ServerConfig(max_batch, max_k, max_nprobe, max_slots, scratch_budget_bytes)
+        cfg = ServerConfig(max_batch, max_k, max_nprobe, max_slots, scratch_budget_bytes,
+                           (ctypes.c_uint32 * 2)(*scan_block))
         h = ctypes.c_void_p()
         check(lib().pdxg_server_create(index.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

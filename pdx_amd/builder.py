@@ -87,6 +87,85 @@ class _Backend:
             out[s:s + chunk] = (at @ bt).cpu().numpy()
         return out
 
+    def preprocess(self, x: np.ndarray, m: np.ndarray, normalize: bool, chunk: int = 1 << 17):
+        """Rows normalised (optionally) and rotated, y = x . M; returns (y, min, max)."""
+        if self.torch is None:
+            y = (normalize_rows(x) if normalize else x) @ m
+            return y.astype(np.float32), float(y.min()), float(y.max())
+        t = self.torch
+        mt = t.from_numpy(np.ascontiguousarray(m)).to(self.dev)
+        y = np.empty(x.shape, np.float32)
+        lo, hi = np.inf, -np.inf
+        for s in range(0, x.shape[0], chunk):
+            xt = t.from_numpy(np.ascontiguousarray(x[s:s + chunk])).to(self.dev)
+            if normalize:
+                nrm = t.linalg.vector_norm(xt, dim=1, keepdim=True)
+                xt = xt / t.where(nrm == 0, t.ones_like(nrm), nrm)
+            yt = xt @ mt
+            lo, hi = min(lo, float(yt.min())), max(hi, float(yt.max()))
+            y[s:s + chunk] = yt.cpu().numpy()
+        return y, lo, hi
+
+    def quantize(self, y: np.ndarray, base: float, scale: float, chunk: int = 1 << 18) -> np.ndarray:
+        if self.torch is None:
+            return quantize(y, base, scale)
+        t = self.torch
+        out = np.empty(y.shape, np.uint8)
+        for s in range(0, y.shape[0], chunk):
+            yt = t.from_numpy(np.ascontiguousarray(y[s:s + chunk])).to(self.dev)
+            v = (yt - np.float32(base)) * np.float32(scale)
+            r = t.sign(v) * t.floor(t.abs(v) + 0.5)
+            out[s:s + chunk] = r.clamp(0, 255).to(t.uint8).cpu().numpy()
+        return out
+
+    def kmeans_resident(self, train: np.ndarray, k: int, iters: int, rng, angular: bool) -> np.ndarray:
+        """Lloyd iterations with the training sample resident on the GPU; returns centroids."""
+        t = self.torch
+        tr = t.from_numpy(np.ascontiguousarray(train)).to(self.dev)
+        n, d = tr.shape
+        cent = self.seed_plusplus(tr, k, rng)
+        chunk = 1 << 16
+        for _ in range(iters):
+            cn = (cent * cent).sum(1)
+            a = t.empty(n, dtype=t.int64, device=self.dev)
+            for s in range(0, n, chunk):
+                a[s:s + chunk] = t.argmin(cn[None, :] - 2.0 * (tr[s:s + chunk] @ cent.T), dim=1)
+            counts = t.bincount(a, minlength=k)
+            sums = t.zeros((k, d), dtype=t.float64, device=self.dev)
+            for s in range(0, n, chunk):
+                sums.index_add_(0, a[s:s + chunk], tr[s:s + chunk].double())
+            empty = counts == 0
+            cent = (sums / counts.clamp(min=1)[:, None].double()).float()
+            ne = int(empty.sum())
+            if ne:
+                cent[empty] = tr[t.from_numpy(rng.choice(n, ne, replace=False)).to(self.dev)]
+            if angular:
+                nrm = t.linalg.vector_norm(cent, dim=1, keepdim=True)
+                cent = cent / t.where(nrm == 0, t.ones_like(nrm), nrm)
+        out = cent.cpu().numpy()
+        del tr
+        self.torch.cuda.empty_cache()
+        return out
+
+    def seed_plusplus(self, tr, k: int, rng, sub: int = 40):
+        """k-means++ seeding (D^2 sampling) over a random subsample of at most sub*k rows (and at
+        most ~128 MB of them)."""
+        t = self.torch
+        n, d = tr.shape
+        m = min(n, sub * k, max(k, (1 << 25) // d))
+        pool = tr[t.from_numpy(np.sort(rng.choice(n, m, replace=False))).to(self.dev)]
+        u = t.from_numpy(rng.random(k)).to(self.dev, t.float64)
+        first = int(rng.integers(0, m))
+        chosen = t.empty(k, dtype=t.int64, device=self.dev)
+        chosen[0] = first
+        d2 = ((pool - pool[first]) ** 2).sum(1).double()
+        for i in range(1, k):
+            cdf = t.cumsum(d2, 0)
+            j = t.searchsorted(cdf, (u[i] * cdf[-1]).reshape(1)).clamp(max=m - 1)
+            chosen[i] = j[0]
+            d2 = t.minimum(d2, ((pool - pool[j]) ** 2).sum(1).double())
+        return pool[chosen].clone()
+
     def cluster_sums(self, x: np.ndarray, a: np.ndarray, k: int) -> np.ndarray:
         """Per-cluster float64 sums of the rows of x grouped by assignment a."""
         if self.torch is None:
@@ -137,7 +216,10 @@ def kmeans(x: np.ndarray, k: int, iters: int, seed: int, sampling_fraction: floa
         return x[:1].copy(), np.zeros(n, np.int64)
     frac = sampling_fraction if sampling_fraction > 0 else (1.0 if n < MIN_EMBEDDINGS_TO_SAMPLE else 0.3)
     ns = max(k, min(n, int(n * frac)))
-    train = x[rng.choice(n, ns, replace=False)] if ns < n else x
+    train = x[np.sort(rng.choice(n, ns, replace=False))] if ns < n else x
+    if backend.torch is not None:
+        cent = backend.kmeans_resident(train, k, iters, rng, angular)
+        return cent, backend.assign(x, cent)
     cent = train[rng.choice(train.shape[0], k, replace=False)].astype(np.float32).copy()
     for _ in range(iters):
         a = backend.assign(train, cent)
@@ -178,13 +260,17 @@ def build_index(data: np.ndarray, index_type: str = "pdx_tree_u8", *, distance_m
     norm = bool(normalize) or distance_metric in ("cosine", "ip")
     be = _Backend(device)
     m = random_rotation(d, seed) if rotation is None else np.ascontiguousarray(rotation, np.float32)
-    y = be.matmul(normalize_rows(x) if norm else x, m)
+    y, lo, hi = be.preprocess(x, m, norm)
     h, v = pf.dimension_split(d)
     u8 = t in (pf.TYPE_U8, pf.TYPE_TREE_U8)
-    base, scale = quantization_params(y) if u8 else (0.0, 1.0)
+    if u8:  # ScalarQuantizer::ComputeQuantizationParams (scalar.hpp:60-74)
+        rng_ = np.float32(np.float32(hi) - np.float32(lo))
+        base, scale = float(np.float32(lo)), float(np.float32(255.0) / rng_ if rng_ > 0 else np.float32(1.0))
+    else:
+        base, scale = 0.0, 1.0
     cent, assign = kmeans(y, nlist, kmeans_iters, seed, sampling_fraction, norm, be)
     L = pf.Level(d, v, h, u8, norm, [], cent, base, scale)
-    stored = quantize(y, base, scale) if u8 else y
+    stored = be.quantize(y, base, scale) if u8 else y
     L.clusters = _clusters_from_assignment(stored, assign, nlist, L, np.arange(n))
     idx = pf.IndexData(type=t, matrix=m, l1=L)
     if t >= 2:

@@ -261,6 +261,7 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
 struct pdxg_workspace {
     int device = 0;
     bool timing = false;
+    DevBuf<unsigned long long> stage_bytes;  // [meso scan, data scan] of the last search when timing
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     ~pdxg_workspace() {
         for (auto& e : ev)
@@ -271,6 +272,7 @@ struct pdxg_workspace {
     }
     uint32_t max_queries = 0, max_k = 0, max_nprobe = 0;
     uint32_t k0_max = 0, order_stride = 0;
+    uint32_t block[2] = {0, 0};  // scan workgroup size per level (meso, data); 0 = heuristic
     DevBuf<float> qnorm, qrot, probe;
     DevBuf<uint8_t> qu8;
     DevBuf<uint32_t> k1, nv1, k0, nv0, order, ids0, cnt0;
@@ -389,14 +391,36 @@ void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_
     HIP_CHECK(hipGetLastError());
 }
 
-// One workgroup per query. Batches too small to give every CU a few workgroups get 1024-thread
-// workgroups (more loads in flight per query); larger batches 256-thread ones (more queries per CU).
-constexpr uint32_t kSmallBatch = 128;
+constexpr uint32_t kBlocks[5] = {64, 128, 256, 512, 1024};
+
+// Workgroup size of a scan launch when none was tuned: about 4 slots per thread for batches that
+// fill the GPU (at most 256 threads, so several queries share a CU), about 2 slots per thread
+// for small batches (more loads in flight per query).
+uint32_t heuristic_block(const Level& L, uint32_t nq) {
+    uint64_t tot = 0;
+    for (uint32_t x : L.n_used) tot += x;
+    const uint32_t mean = (uint32_t)std::max<uint64_t>(1, tot / std::max(1u, L.nc));
+    const bool small = nq < 64;
+    uint32_t want = 64;
+    while (want < (small ? mean / 2 : mean / 4) && want < (small ? 1024u : 256u)) want <<= 1;
+    return want;
+}
 
 template <bool U8, bool FILT>
-void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
-    if (a.nq < kSmallBatch) launch_scan_block<U8, FILT, 1024>(ix, L, a, kmax, st);
-    else launch_scan_block<U8, FILT, 256>(ix, L, a, kmax, st);
+void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st, uint32_t block = 0) {
+    if (block == 0) block = heuristic_block(L, a.nq);
+    if constexpr (FILT) {  // filtered search: single queries, two sizes
+        if (block >= 512) launch_scan_block<U8, FILT, 1024>(ix, L, a, kmax, st);
+        else launch_scan_block<U8, FILT, 256>(ix, L, a, kmax, st);
+    } else {
+        switch (block) {
+            case 64: launch_scan_block<U8, FILT, 64>(ix, L, a, kmax, st); break;
+            case 128: launch_scan_block<U8, FILT, 128>(ix, L, a, kmax, st); break;
+            case 256: launch_scan_block<U8, FILT, 256>(ix, L, a, kmax, st); break;
+            case 512: launch_scan_block<U8, FILT, 512>(ix, L, a, kmax, st); break;
+            default: launch_scan_block<U8, FILT, 1024>(ix, L, a, kmax, st); break;
+        }
+    }
 }
 
 void launch_probe(const pdxg_index* ix, const Level& L, const float* qrot, uint32_t nq, const uint32_t* nvisit,
@@ -459,6 +483,13 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
     if (nq == 0) return;
 
     ws.mark(0, st);
+    unsigned long long* sb = nullptr;
+    if (ws.timing) {
+        if (!ws.stage_bytes.p) ws.stage_bytes.alloc(2);
+        sb = ws.stage_bytes.p;
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, sb, 2u, 0ull);
+        HIP_CHECK(hipGetLastError());
+    }
     launch_preprocess(ix, ws, a.queries, nq, a.queries_transformed != 0, st);
 
     SetupArgs sa{};
@@ -508,8 +539,9 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s.out_idists = a.out_int_distances;
         s.out_counts = a.out_counts;
         s.out_stride = a.out_stride;
-        if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s, a.k_max, st);
-        else launch_scan<false, false>(ix, ix->l1, s, a.k_max, st);
+        s.stage_bytes = sb ? sb + 1 : nullptr;
+        if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s, a.k_max, st, ws.block[1]);
+        else launch_scan<false, false>(ix, ix->l1, s, a.k_max, st, ws.block[1]);
         ws.mark(4, st);
     } else {
         require(ws.k0_max <= kMaxK,
@@ -529,7 +561,8 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s0.out_idists = nullptr;
         s0.out_counts = ws.cnt0.p;
         s0.out_stride = ws.k0_max;
-        launch_scan<false, false>(ix, ix->l0, s0, ws.k0_max, st);
+        s0.stage_bytes = sb;
+        launch_scan<false, false>(ix, ix->l0, s0, ws.k0_max, st, ws.block[0]);
         ws.mark(3, st);
         ScanArgs s1 = base;
         s1.L = ix->l1.view(ix->qscale_sq, ix->inv_qscale_sq);
@@ -545,8 +578,9 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s1.out_idists = a.out_int_distances;
         s1.out_counts = a.out_counts;
         s1.out_stride = a.out_stride;
-        if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s1, a.k_max, st);
-        else launch_scan<false, false>(ix, ix->l1, s1, a.k_max, st);
+        s1.stage_bytes = sb ? sb + 1 : nullptr;
+        if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s1, a.k_max, st, ws.block[1]);
+        else launch_scan<false, false>(ix, ix->l1, s1, a.k_max, st, ws.block[1]);
         ws.mark(4, st);
     }
 }
@@ -713,6 +747,77 @@ pdxg_status pdxg_workspace_stage_times(pdxg_workspace* ws, float* out) {
         DeviceGuard g(ws->device);
         HIP_CHECK(hipEventSynchronize(ws->ev[4]));
         for (int i = 0; i < 4; ++i) HIP_CHECK(hipEventElapsedTime(&out[i], ws->ev[i], ws->ev[i + 1]));
+    });
+}
+
+pdxg_status pdxg_workspace_stage_bytes(pdxg_workspace* ws, uint64_t* out) {
+    return guarded([&] {
+        require(ws && out, "null argument");
+        require(ws->timing && ws->stage_bytes.p, "timing is not enabled on this workspace");
+        DeviceGuard g(ws->device);
+        HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+        HIP_CHECK(hipMemcpy(out, ws->stage_bytes.p, 16, hipMemcpyDeviceToHost));
+    });
+}
+
+pdxg_status pdxg_workspace_set_plan(pdxg_workspace* ws, uint32_t block_meso, uint32_t block_data) {
+    return guarded([&] {
+        require(ws != nullptr, "null argument");
+        auto ok = [](uint32_t b) { return b == 0 || b == 64 || b == 128 || b == 256 || b == 512 || b == 1024; };
+        require(ok(block_meso) && ok(block_data), "scan block sizes must be 0, 64, 128, 256, 512 or 1024");
+        ws->block[0] = block_meso;
+        ws->block[1] = block_data;
+    });
+}
+
+pdxg_status pdxg_workspace_get_plan(pdxg_workspace* ws, uint32_t* out2) {
+    return guarded([&] {
+        require(ws && out2, "null argument");
+        out2[0] = ws->block[0];
+        out2[1] = ws->block[1];
+    });
+}
+
+pdxg_status pdxg_workspace_autotune(pdxg_index* ix, pdxg_workspace* ws, const pdxg_search_args* args, void* stream) {
+    return guarded([&] {
+        require(ix && ws && args, "null argument");
+        DeviceGuard g(ix->device);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const bool was_timing = ws->timing;
+        if (!was_timing) {
+            for (auto& e : ws->ev)
+                if (!e) HIP_CHECK(hipEventCreate(&e));
+            ws->timing = true;
+        }
+        // tune the meso level first (two-level indexes), then the data level; each candidate runs
+        // twice and the second run's stage time counts (results do not depend on the block size)
+        for (int lvl = ix->tree() ? 0 : 1; lvl <= 1; ++lvl) {
+            float best = 1e30f;
+            uint32_t best_b = 0;
+            for (uint32_t b : kBlocks) {
+                ws->block[lvl] = b;
+                float ms[4] = {0, 0, 0, 0};
+                bool ok = true;
+                for (int rep = 0; rep < 2 && ok; ++rep) {
+                    try {
+                        search_device(ix, *ws, *args, st);
+                        HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+                        for (int i = 0; i < 4; ++i) HIP_CHECK(hipEventElapsedTime(&ms[i], ws->ev[i], ws->ev[i + 1]));
+                    } catch (const Error& e) {
+                        if (e.code != PDXG_ERR_UNSUPPORTED) throw;
+                        ok = false;  // e.g. the LDS plan of this block size does not fit
+                    }
+                }
+                const float t = lvl == 0 ? ms[2] : ms[3];
+                if (ok && t < best) {
+                    best = t;
+                    best_b = b;
+                }
+            }
+            require(best_b != 0, "no scan block size fits this index", PDXG_ERR_UNSUPPORTED);
+            ws->block[lvl] = best_b;
+        }
+        ws->timing = was_timing;
     });
 }
 

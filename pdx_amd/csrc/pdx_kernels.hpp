@@ -52,6 +52,7 @@ struct ScanArgs {
     uint32_t* out_counts;
     uint32_t out_stride;
     unsigned long long* out_bytes;
+    unsigned long long* stage_bytes;  // optional: algorithmic bytes of this launch, summed over queries
     const uint8_t* sel;         // filtered search: selection byte per slot (cluster at slot_off)
     const uint32_t* n_pass;     // filtered search: passing slots per cluster
     uint32_t nq;
@@ -116,6 +117,11 @@ __global__ void k_setup(SetupArgs a) {
     a.k0[q] = np1 > a.k0_cap ? a.k0_cap : np1;
     a.nvisit0[q] = (np0 == 0 || np0 > a.nc0) ? a.nc0 : np0;
     a.nvisit1[q] = np1 > a.nc1 ? a.nc1 : np1;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -778,6 +784,7 @@ struct Scanner {
         if (threadIdx.x == 0) {
             a.out_counts[q] = cnt;
             if (a.out_bytes) a.out_bytes[q] += bytes;
+            if (a.stage_bytes) atomicAdd(a.stage_bytes, bytes);
         }
     }
 };

@@ -96,3 +96,35 @@ def test_pre_rotated_queries(gpu_lib):
     g = gpu_lib.GpuIndex(pf.load(path))
     qr = o.preprocess(q)
     _compare(o.search(qr, 10, 5, transformed=True), g.search(qr, 10, 5, transformed=True, want_bytes=True), True)
+
+
+@pytest.mark.parametrize("kind", ["pdx_tree_u8", "pdx_f32"])
+def test_every_scan_block_size_gives_the_oracle_results(gpu_lib, kind):
+    import torch
+
+    path, x, q = index_file(kind, 4000, 768, 40) if kind == "pdx_tree_u8" else index_file(kind, 2500, 384, 20, normalize=True)
+    o = OracleIndex(path)
+    g = gpu_lib.GpuIndex(pf.load(path))
+    nq, K, npb = len(q), 20, 6
+    oi, od, oid, oc, _ = o.search(q, K, npb)
+    dev = torch.device("cuda", 0)
+    qd = torch.from_numpy(q).to(dev)
+    ids = torch.zeros((nq, K), dtype=torch.int32, device=dev)
+    dd = torch.zeros((nq, K), dtype=torch.float32, device=dev)
+    cnt = torch.zeros(nq, dtype=torch.int32, device=dev)
+    ws = gpu_lib.Workspace(g, nq, K, npb)
+    sp = torch.cuda.current_stream().cuda_stream
+    for b in (64, 128, 256, 512, 1024):
+        print(f"{kind} block {b}", flush=True)
+        ws.set_plan(b, b)
+        ws.search(qd.data_ptr(), nq, 0, 0, K, npb, K, ids.data_ptr(), dd.data_ptr(), 0, cnt.data_ptr(), 0,
+                  stream=sp, k_all=K, nprobe_all=npb)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(cnt.cpu().numpy().view(np.uint32), oc)
+        np.testing.assert_array_equal(ids.cpu().numpy().view(np.uint32), oi)
+        np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), od.view(np.uint32))
+    plan = ws.autotune(qd.data_ptr(), nq, 0, 0, K, npb, K, ids.data_ptr(), dd.data_ptr(), 0, cnt.data_ptr(),
+                       stream=sp, k_all=K, nprobe_all=npb)
+    assert plan[1] in (64, 128, 256, 512, 1024)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ids.cpu().numpy().view(np.uint32), oi)
