@@ -137,6 +137,15 @@ pdxg_status pdxg_workspace_stage_bytes(pdxg_workspace* ws, uint64_t* out2);
  * reference. */
 pdxg_status pdxg_workspace_set_plan(pdxg_workspace* ws, uint32_t block_meso, uint32_t block_data);
 pdxg_status pdxg_workspace_get_plan(pdxg_workspace* ws, uint32_t* out2);
+/* Two-phase meso-level scan of two-level indexes (on by default where the workspace holds its
+ * summaries): phase 1 computes every partial distance the meso scan can compare, for all
+ * (query, visited meso member) pairs, sharing each meso cluster's reads across the queries that
+ * visit it; phase 2 replays the reference's Warmup / Prune / Merge on those values. Results are
+ * identical either way; the autotuner also picks between the two. pdxg_workspace_meso_summary
+ * returns 1 when the next search will use it (tombstoned meso members or meso clusters grown past
+ * the workspace's plan fall back to the pruned scan). Not in the reference. */
+pdxg_status pdxg_workspace_set_meso_summary(pdxg_workspace* ws, int enable);
+int pdxg_workspace_meso_summary(const pdxg_workspace* ws);
 pdxg_status pdxg_workspace_autotune(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,
                                     void* stream);
 

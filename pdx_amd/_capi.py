@@ -77,7 +77,8 @@ EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster
            "pdxg_search_device", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
            "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times",
            "pdxg_workspace_stage_bytes",
-           "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
+           "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune",
+           "pdxg_workspace_set_meso_summary", "pdxg_workspace_meso_summary", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
            "pdxg_server_get_stats", "pdxg_server_replay"]
 
 _lib = None
@@ -114,6 +115,10 @@ def lib():
     L.pdxg_workspace_set_plan.argtypes = [vp, u32, u32]
     L.pdxg_workspace_get_plan.argtypes = [vp, u32p]
     L.pdxg_workspace_autotune.argtypes = [vp, vp, ctypes.POINTER(SearchArgs), vp]
+    L.pdxg_workspace_set_meso_summary.argtypes = [vp, ctypes.c_int]
+    L.pdxg_workspace_set_meso_summary.restype = ctypes.c_int
+    L.pdxg_workspace_meso_summary.argtypes = [vp]
+    L.pdxg_workspace_meso_summary.restype = ctypes.c_int
     u64p, dblp = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)
     L.pdxg_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     L.pdxg_server_destroy.argtypes = [vp]
@@ -292,6 +297,13 @@ class Workspace:
 
     def set_plan(self, block_meso: int = 0, block_data: int = 0) -> None:
         check(lib().pdxg_workspace_set_plan(self._h, block_meso, block_data))
+
+    def set_meso_summary(self, enable: bool) -> None:
+        check(lib().pdxg_workspace_set_meso_summary(self._h, int(enable)))
+
+    @property
+    def meso_summary(self) -> bool:
+        return bool(lib().pdxg_workspace_meso_summary(self._h))
 
     def plan(self):
         out = np.zeros(2, np.uint32)

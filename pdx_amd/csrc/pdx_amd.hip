@@ -14,6 +14,7 @@
 
 #include "../../include/pdx_amd.h"
 #include "pdx_kernels.hpp"
+#include "pdx_meso.hpp"
 
 namespace {
 
@@ -100,6 +101,8 @@ struct Level {
     std::vector<uint32_t> n_used, n_live, cap, n_tomb;
     std::vector<uint64_t> data_off, slot_off, tomb_off;
     uint64_t data_bytes = 0, total_slots = 0, tomb_words = 0;
+    uint64_t tomb_total = 0;
+    std::vector<uint64_t> top_prefix;  // [m] = sum of the m largest n_used
     DevBuf<uint8_t> data;
     DevBuf<uint64_t> d_data_off, d_slot_off, d_tomb_off;
     DevBuf<uint32_t> ids, d_n_used, d_n_live, d_cap, tomb, d_n_tomb;
@@ -198,6 +201,12 @@ struct Level {
         d_n_used.upload(n_used.data(), nc);
         d_n_live.upload(n_live.data(), nc);
         d_n_tomb.upload(n_tomb.data(), nc);
+        tomb_total = 0;
+        for (uint32_t x : n_tomb) tomb_total += x;
+        std::vector<uint32_t> sz(n_used);
+        std::sort(sz.begin(), sz.end(), std::greater<uint32_t>());
+        top_prefix.assign(nc + 1, 0);
+        for (uint32_t c = 0; c < nc; ++c) top_prefix[c + 1] = top_prefix[c] + sz[c];
     }
 
     LevelDev view(float scale_sq, float inv_scale_sq) const {
@@ -225,7 +234,7 @@ struct Level {
 
 struct LdsPlan {
     uint32_t bytes = 0;
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red;
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau;
 };
 
 LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
@@ -252,6 +261,7 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
     p.o_op = take(kmax * 4);
     p.o_cp = take(block * 4);
     p.o_red = take(64);
+    p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + kMaxCheckpoints) * 4);
     p.bytes = (off + 15) / 16 * 16;
     return p;
 }
@@ -276,9 +286,18 @@ struct pdxg_workspace {
     DevBuf<float> qnorm, qrot, probe;
     DevBuf<uint8_t> qu8;
     DevBuf<uint32_t> k1, nv1, k0, nv0, order, ids0, cnt0;
+    // two-phase meso scan (two-level indexes): partial-distance summaries of the visited meso
+    // clusters, `region` vectors per query
+    bool summ_ok = false;   // buffers allocated
+    bool summ_use = true;   // plan: use them when the index allows
+    uint32_t nv0_max = 0, nw = 0, ns = 0;
+    unsigned long long region = 0;
+    DevBuf<uint32_t> m_cnt, m_cur, m_lstart, m_cstart, m_list, m_soff;
+    DevBuf<float> summ;
     size_t bytes() const {
         return qnorm.bytes() + qrot.bytes() + probe.bytes() + qu8.bytes() + k1.bytes() + nv1.bytes() + k0.bytes() +
-               nv0.bytes() + order.bytes() + ids0.bytes() + cnt0.bytes();
+               nv0.bytes() + order.bytes() + ids0.bytes() + cnt0.bytes() + m_cnt.bytes() + m_cur.bytes() +
+               m_lstart.bytes() + m_cstart.bytes() + m_list.bytes() + m_soff.bytes() + summ.bytes();
     }
 };
 
@@ -322,12 +341,40 @@ uint32_t k0_capacity(const pdxg_index* ix, uint32_t max_nprobe) {
     return (max_nprobe == 0 || max_nprobe > ix->l1.nc) ? ix->l1.nc : max_nprobe;
 }
 
+// Summaries larger than this per workspace fall back to the pruned meso scan.
+constexpr size_t kMaxSummaryBytes = 24ull << 30;
+
+// Warmup checkpoints of a scan over v vertical dims (the cumulative fetch sizes clipped to v).
+uint32_t warmup_checkpoints(uint32_t v) {
+    static const uint32_t fetch[20] = {16, 16, 32, 32, 32, 32, 64, 64, 64, 64,
+                                       128, 128, 128, 128, 256, 256, 512, 1024, 2048, 16384};
+    uint32_t n = 0, end = 0;
+    while (end < v) {
+        require(n < 20, "vertical dimensions beyond the warmup schedule");
+        end = std::min(end + fetch[n], v);
+        ++n;
+    }
+    return n;
+}
+
 void plan_workspace(const pdxg_index* ix, uint32_t nq, uint32_t max_k, uint32_t max_nprobe, pdxg_workspace& ws) {
     ws.max_queries = nq;
     ws.max_k = max_k;
     ws.max_nprobe = max_nprobe;
     ws.k0_max = k0_capacity(ix, max_nprobe);
     ws.order_stride = ix->tree() ? ix->l0.nc : ((max_nprobe == 0 || max_nprobe > ix->l1.nc) ? ix->l1.nc : max_nprobe);
+    ws.summ_ok = false;
+    if (ix->tree()) {
+        // meso clusters visited per query (k_setup): all of them, or half when nprobe < nlist / 2
+        const uint32_t n0 = ix->l0.nc, n1 = ix->l1.nc;
+        const bool halve = max_nprobe != 0 && max_nprobe < n1 / 2 && n0 / 2 != 0;
+        ws.nv0_max = halve ? n0 / 2 : n0;
+        ws.region = ix->l0.top_prefix[ws.nv0_max];
+        ws.nw = warmup_checkpoints(ix->l0.v);
+        ws.ns = ws.nw + ix->l0.h / kHDim;
+        ws.summ_ok = (size_t)nq * ws.region * ws.ns * 4 <= kMaxSummaryBytes && (uint64_t)nq * ws.region < (1ull << 32) &&
+                     (uint64_t)nq * ix->d < (1ull << 32);
+    }
 }
 
 size_t workspace_bytes_for(const pdxg_index* ix, uint32_t nq, uint32_t max_k, uint32_t max_nprobe) {
@@ -342,6 +389,12 @@ size_t workspace_bytes_for(const pdxg_index* ix, uint32_t nq, uint32_t max_k, ui
     b += 4 * (size_t)nq * 4;               // k1 nv1 k0 nv0
     b += (size_t)nq * ws.order_stride * 4; // order
     b += (size_t)nq * ws.k0_max * 4 + (size_t)nq * 4;
+    if (ws.summ_ok) {
+        b += 4 * ((size_t)ix->l0.nc + 1) * 4;        // counts, cursors, entry / item offsets
+        b += (size_t)nq * ws.nv0_max * 4;            // entries
+        b += (size_t)nq * ws.order_stride * 4;       // summary offsets
+        b += (size_t)nq * ws.region * ws.ns * 4;     // summaries
+    }
     return b;
 }
 
@@ -361,9 +414,19 @@ void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
         ws.ids0.alloc(nq * ws.k0_max);
         ws.cnt0.alloc(nq);
     }
+    if (ws.summ_ok) {
+        const size_t n0 = ix->l0.nc;
+        ws.m_cnt.alloc(n0 + 1);
+        ws.m_cur.alloc(n0 + 1);
+        ws.m_lstart.alloc(n0 + 1);
+        ws.m_cstart.alloc(n0 + 1);
+        ws.m_list.alloc(nq * ws.nv0_max);
+        ws.m_soff.alloc(nq * ws.order_stride);
+        ws.summ.alloc(nq * ws.region * ws.ns);
+    }
 }
 
-template <bool U8, bool FILT, uint32_t BLOCK>
+template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false>
 void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
     LdsPlan p = plan_scan_lds(L, kmax, BLOCK);
     require(p.bytes <= (uint32_t)ix->max_lds,
@@ -382,7 +445,8 @@ void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_
     a.o_ck = p.o_ck;
     a.o_cp = p.o_cp;
     a.o_red = p.o_red;
-    auto kern = k_scan<U8, FILT, BLOCK>;
+    a.o_tau = p.o_tau;
+    auto kern = k_scan<U8, FILT, BLOCK, SUMM>;
     if (p.bytes > 65536)
         HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)p.bytes));
@@ -406,7 +470,7 @@ uint32_t heuristic_block(const Level& L, uint32_t nq) {
     return want;
 }
 
-template <bool U8, bool FILT>
+template <bool U8, bool FILT, bool SUMM = false>
 void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st, uint32_t block = 0) {
     if (block == 0) block = heuristic_block(L, a.nq);
     if constexpr (FILT) {  // filtered search: single queries, two sizes
@@ -414,13 +478,51 @@ void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax
         else launch_scan_block<U8, FILT, 256>(ix, L, a, kmax, st);
     } else {
         switch (block) {
-            case 64: launch_scan_block<U8, FILT, 64>(ix, L, a, kmax, st); break;
-            case 128: launch_scan_block<U8, FILT, 128>(ix, L, a, kmax, st); break;
-            case 256: launch_scan_block<U8, FILT, 256>(ix, L, a, kmax, st); break;
-            case 512: launch_scan_block<U8, FILT, 512>(ix, L, a, kmax, st); break;
-            default: launch_scan_block<U8, FILT, 1024>(ix, L, a, kmax, st); break;
+            case 64: launch_scan_block<U8, FILT, 64, SUMM>(ix, L, a, kmax, st); break;
+            case 128: launch_scan_block<U8, FILT, 128, SUMM>(ix, L, a, kmax, st); break;
+            case 256: launch_scan_block<U8, FILT, 256, SUMM>(ix, L, a, kmax, st); break;
+            case 512: launch_scan_block<U8, FILT, 512, SUMM>(ix, L, a, kmax, st); break;
+            default: launch_scan_block<U8, FILT, 1024, SUMM>(ix, L, a, kmax, st); break;
         }
     }
+}
+
+// Phase 1 of the two-phase meso scan: visit lists inverted per meso cluster, then the
+// partial-distance summaries of every visited (query, meso member) pair.
+void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, hipStream_t st) {
+    const Level& L = ix->l0;
+    MesoArgs m{};
+    m.L = L.view(1.0f, 1.0f);
+    m.qf = ws.qrot.p;
+    m.order = ws.order.p;
+    m.nvisit = ws.nv0.p;
+    m.stride = ws.order_stride;
+    m.nq = nq;
+    m.nv_max = ws.nv0_max;
+    m.cnt = ws.m_cnt.p;
+    m.cur = ws.m_cur.p;
+    m.lstart = ws.m_lstart.p;
+    m.cstart = ws.m_cstart.p;
+    m.list = ws.m_list.p;
+    m.soff = ws.m_soff.p;
+    m.summ = ws.summ.p;
+    m.region = ws.region;
+    m.nw = ws.nw;
+    m.ns = ws.ns;
+    require((uint64_t)nq * ws.order_stride < (1ull << 32), "too many queries x meso clusters for one search");
+    hipLaunchKernelGGL(k_fill_u32, dim3((L.nc + 255) / 256), dim3(256), 0, st, ws.m_cnt.p, L.nc, 0u);
+    const size_t pairs = (size_t)nq * ws.nv0_max;
+    const dim3 gp((unsigned)((pairs + 255) / 256));
+    hipLaunchKernelGGL(k_meso_count, gp, dim3(256), 0, st, m);
+    hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(1024), 0, st, m);
+    hipLaunchKernelGGL(k_meso_scatter, gp, dim3(256), 0, st, m);
+    hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(256), 0, st, m);
+    // work items: sum over clusters of ceil(entries / kMesoQ) <= pairs / kMesoQ + clusters
+    const size_t items = pairs / kMesoQ + L.nc;
+    const size_t wgs = (items + kMesoWaves - 1) / kMesoWaves;
+    m.wg_per_xcd = (uint32_t)((wgs + 7) / 8);
+    hipLaunchKernelGGL(k_meso_summary, dim3(8 * m.wg_per_xcd), dim3(kMesoWaves * kWave), 0, st, m);
+    HIP_CHECK(hipGetLastError());
 }
 
 void launch_probe(const pdxg_index* ix, const Level& L, const float* qrot, uint32_t nq, const uint32_t* nvisit,
@@ -502,6 +604,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
     sa.nc1 = ix->l1.nc;
     sa.nc0 = ix->tree() ? ix->l0.nc : 0;
     sa.k_cap = a.k_max;
+    sa.np_cap = np_cap;
     sa.order_cap = ws.order_stride;
     sa.k0_cap = ws.k0_max;
     sa.k1 = ws.k1.p;
@@ -562,7 +665,19 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s0.out_counts = ws.cnt0.p;
         s0.out_stride = ws.k0_max;
         s0.stage_bytes = sb;
-        launch_scan<false, false>(ix, ix->l0, s0, ws.k0_max, st, ws.block[0]);
+        const bool summ = ws.summ_ok && ws.summ_use && ix->l0.tomb_total == 0 &&
+                          ix->l0.top_prefix[ws.nv0_max] <= ws.region;
+        if (summ) {
+            launch_meso_summary(ix, ws, nq, st);
+            s0.summ = ws.summ.p;
+            s0.soff = ws.m_soff.p;
+            s0.region = ws.region;
+            s0.nw = ws.nw;
+            s0.ns = ws.ns;
+            launch_scan<false, false, true>(ix, ix->l0, s0, ws.k0_max, st, ws.block[0]);
+        } else {
+            launch_scan<false, false>(ix, ix->l0, s0, ws.k0_max, st, ws.block[0]);
+        }
         ws.mark(3, st);
         ScanArgs s1 = base;
         s1.L = ix->l1.view(ix->qscale_sq, ix->inv_qscale_sq);
@@ -770,6 +885,15 @@ pdxg_status pdxg_workspace_set_plan(pdxg_workspace* ws, uint32_t block_meso, uin
     });
 }
 
+pdxg_status pdxg_workspace_set_meso_summary(pdxg_workspace* ws, int enable) {
+    return guarded([&] {
+        require(ws != nullptr, "null argument");
+        ws->summ_use = enable != 0;
+    });
+}
+
+int pdxg_workspace_meso_summary(const pdxg_workspace* ws) { return ws && ws->summ_ok && ws->summ_use ? 1 : 0; }
+
 pdxg_status pdxg_workspace_get_plan(pdxg_workspace* ws, uint32_t* out2) {
     return guarded([&] {
         require(ws && out2, "null argument");
@@ -791,31 +915,39 @@ pdxg_status pdxg_workspace_autotune(pdxg_index* ix, pdxg_workspace* ws, const pd
         }
         // tune the meso level first (two-level indexes), then the data level; each candidate runs
         // twice and the second run's stage time counts (results do not depend on the block size)
+        // (the meso level also tries the two-phase scan against the pruned one)
         for (int lvl = ix->tree() ? 0 : 1; lvl <= 1; ++lvl) {
             float best = 1e30f;
             uint32_t best_b = 0;
-            for (uint32_t b : kBlocks) {
-                ws->block[lvl] = b;
-                float ms[4] = {0, 0, 0, 0};
-                bool ok = true;
-                for (int rep = 0; rep < 2 && ok; ++rep) {
-                    try {
-                        search_device(ix, *ws, *args, st);
-                        HIP_CHECK(hipEventSynchronize(ws->ev[4]));
-                        for (int i = 0; i < 4; ++i) HIP_CHECK(hipEventElapsedTime(&ms[i], ws->ev[i], ws->ev[i + 1]));
-                    } catch (const Error& e) {
-                        if (e.code != PDXG_ERR_UNSUPPORTED) throw;
-                        ok = false;  // e.g. the LDS plan of this block size does not fit
+            bool best_summ = false;
+            for (int summ = (lvl == 0 && ws->summ_ok) ? 1 : 0; summ >= 0; --summ) {
+                ws->summ_use = summ != 0;
+                for (uint32_t b : kBlocks) {
+                    ws->block[lvl] = b;
+                    float ms[4] = {0, 0, 0, 0};
+                    bool ok = true;
+                    for (int rep = 0; rep < 2 && ok; ++rep) {
+                        try {
+                            search_device(ix, *ws, *args, st);
+                            HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+                            for (int i = 0; i < 4; ++i)
+                                HIP_CHECK(hipEventElapsedTime(&ms[i], ws->ev[i], ws->ev[i + 1]));
+                        } catch (const Error& e) {
+                            if (e.code != PDXG_ERR_UNSUPPORTED) throw;
+                            ok = false;  // e.g. the LDS plan of this block size does not fit
+                        }
                     }
-                }
-                const float t = lvl == 0 ? ms[2] : ms[3];
-                if (ok && t < best) {
-                    best = t;
-                    best_b = b;
+                    const float t = lvl == 0 ? ms[2] : ms[3];
+                    if (ok && t < best) {
+                        best = t;
+                        best_b = b;
+                        best_summ = summ != 0;
+                    }
                 }
             }
             require(best_b != 0, "no scan block size fits this index", PDXG_ERR_UNSUPPORTED);
             ws->block[lvl] = best_b;
+            if (lvl == 0) ws->summ_use = best_summ;
         }
         ws->timing = was_timing;
     });

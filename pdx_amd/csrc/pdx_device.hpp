@@ -48,6 +48,16 @@ __device__ __forceinline__ float tree16(float acc) {
     acc = acc + row_ror<14>(acc);  // + lane l+2
     return acc;
 }
+// The same tree for 16 per-lane partial sums held by one thread (lane l's sum in p[l]); equal to
+// lane 0 of tree16 bit for bit.
+__device__ __forceinline__ float tree16_regs(const float (&p)[16]) {
+    float x[8], r[4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = p[i] + p[i + 8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = x[i] + x[i + 4];
+    return (r[0] + r[1]) + (r[2] + r[3]);
+}
 __device__ __forceinline__ uint32_t sum16_u(uint32_t acc) {
     acc += row_ror_u<8>(acc);
     acc += row_ror_u<4>(acc);
@@ -102,6 +112,31 @@ __device__ __forceinline__ uint32_t block_excl_scan(bool p, uint32_t* s_red, uin
     __syncthreads();
     *total = tot;
     return off + r;
+}
+
+// Exclusive prefix sum of x in thread order; *total receives the block-wide sum.
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t x, uint32_t* s_red, uint32_t* total) {
+    constexpr uint32_t NW = BLOCK / kWave;
+    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t inc = x;
+#pragma unroll
+    for (uint32_t off = 1; off < kWave; off <<= 1) {
+        uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+    }
+    if (lane == kWave - 1) s_red[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < NW; ++i) {
+        uint32_t c = s_red[i];
+        off += (i < w) ? c : 0u;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - x;
 }
 
 // In-LDS bitonic sort (ascending) of n2 (power of two) keys with payloads.
@@ -165,6 +200,8 @@ __device__ __forceinline__ uint32_t merge_sorted(const uint64_t* a, const uint32
     return n < cap ? n : cap;
 }
 
+constexpr uint32_t kRankSortMax = 256;
+
 // Bounded sorted list in LDS (keys L / payload LP, length len, capacity cap) receiving one
 // optional candidate per thread. Keeps the `cap` smallest keys seen -- identical to pushing the
 // candidates one by one into a max-heap of size cap with a strict-less replacement test
@@ -179,18 +216,37 @@ __device__ __forceinline__ uint32_t list_insert_chunk(bool valid, uint64_t key, 
     uint32_t s = 0;
     uint32_t r = block_excl_scan<BLOCK>(valid, s_red, &s);
     if (s == 0) return len;
-    uint32_t n2 = 1;
-    while (n2 < s) n2 <<= 1;
-    if (valid) {
-        ck[r] = key;
-        cp[r] = pay;
+    if (s <= kRankSortMax) {
+        // few candidates: each one's rank is the number of smaller keys (keys are unique), then
+        // an in-place scatter; 3 barriers instead of the sort network's log^2 stages
+        if (valid) {
+            ck[r] = key;
+            cp[r] = pay;
+        }
+        __syncthreads();
+        uint32_t rank = 0;
+        if (valid)
+            for (uint32_t t = 0; t < s; ++t) rank += ck[t] < key;
+        __syncthreads();
+        if (valid) {
+            ck[rank] = key;
+            cp[rank] = pay;
+        }
+        __syncthreads();
+    } else {
+        uint32_t n2 = 1;
+        while (n2 < s) n2 <<= 1;
+        if (valid) {
+            ck[r] = key;
+            cp[r] = pay;
+        }
+        for (uint32_t i = s + threadIdx.x; i < n2; i += BLOCK) {
+            ck[i] = kKeyInf;
+            cp[i] = 0;
+        }
+        __syncthreads();
+        bitonic_sort<BLOCK>(ck, cp, n2);
     }
-    for (uint32_t i = s + threadIdx.x; i < n2; i += BLOCK) {
-        ck[i] = kKeyInf;
-        cp[i] = 0;
-    }
-    __syncthreads();
-    bitonic_sort<BLOCK>(ck, cp, n2);
     uint32_t nl = merge_sorted<BLOCK>(L, LP, len, ck, cp, s, ok, op, cap);
     for (uint32_t i = threadIdx.x; i < nl; i += BLOCK) {
         L[i] = ok[i];

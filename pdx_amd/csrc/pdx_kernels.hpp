@@ -55,10 +55,18 @@ struct ScanArgs {
     unsigned long long* stage_bytes;  // optional: algorithmic bytes of this launch, summed over queries
     const uint8_t* sel;         // filtered search: selection byte per slot (cluster at slot_off)
     const uint32_t* n_pass;     // filtered search: passing slots per cluster
+    // two-phase meso scan: per (query, visited cluster) partial-distance summaries
+    const float* summ;          // regions of `region` vectors x ns floats per query
+    const uint32_t* soff;       // [nq x order_stride] first summarised vector of (query, rank)
+    unsigned long long region;
+    uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector (nw + h/64)
     uint32_t nq;
     // dynamic LDS layout (byte offsets)
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red;
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau;
 };
+
+// Largest number of warmup checkpoints (entries of DIMENSIONS_FETCHING_SIZES used).
+constexpr uint32_t kMaxCheckpoints = 20;
 
 template <uint32_t BLOCK>
 __device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t* s_red) {
@@ -85,6 +93,7 @@ struct SetupArgs {
     uint32_t tree;
     uint32_t nc1, nc0;
     uint32_t k_cap;      // largest k the scan buffers hold
+    uint32_t np_cap;     // declared largest nprobe (0: unbounded)
     uint32_t order_cap;  // flat: row length of the probe order buffer
     uint32_t k0_cap;     // tree: row length of the L0 result buffer
     uint32_t* k1;        // final k
@@ -102,6 +111,7 @@ __global__ void k_setup(SetupArgs a) {
     uint32_t k = a.k_in ? a.k_in[q] : a.k_all;
     k = k < 1 ? 1 : (k > a.k_cap ? a.k_cap : k);
     uint32_t np = a.nprobe_in ? a.nprobe_in[q] : a.nprobe_all;
+    if (a.np_cap != 0 && (np == 0 || np > a.np_cap)) np = a.np_cap;
     a.k1[q] = k;
     if (a.bytes) a.bytes[q] = a.probe_bytes;
     if (!a.tree) {  // searcher.hpp:614-616
@@ -272,8 +282,13 @@ struct DistT<true> {
 
 constexpr uint32_t kVT = 4;  // vectors per thread per vertical tile
 
-template <bool U8, bool FILT, uint32_t BLOCK>
+// SUMM: the meso-level replay. Every partial distance the scan compares (warmup checkpoints
+// and 64-dim horizontal blocks) was computed beforehand by k_meso_summary in the same operation
+// order, so the replay reads those values instead of the vectors; the vertical remainder after
+// an early warmup exit still reads the vectors.
+template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false>
 struct Scanner {
+    static_assert(!SUMM || (!U8 && !FILT), "the summary replay is F32 and unfiltered");
     using dist_t = typename DistT<U8>::type;
     static constexpr uint32_t G = BLOCK / 16;  // 16-lane rows per workgroup
 
@@ -286,16 +301,21 @@ struct Scanner {
     uint32_t* pos;
     uint64_t *H, *T, *O, *CK;
     uint32_t *HP, *TP, *OP, *CP, *red;
+    dist_t* taus;     // thresholds of the current prune steps
+    uint32_t* wcnt;   // SUMM: pruned counts per warmup checkpoint
     uint32_t d, v, h, nvw;
     uint32_t hl = 0;
-    unsigned long long bytes = 0;
+    unsigned long long bytes = 0;        // uniform: identical in every thread
+    unsigned long long bytes_local = 0;  // this thread's share (summed over the block at the end)
 
     // per-cluster state
     const uint8_t* base;
     const uint32_t* cid;
     const uint32_t* tbm;
     const uint8_t* csel;
+    const float* sb;  // SUMM: summary block of the current (query, cluster), component-major
     uint32_t n, S, ntomb;
+    bool hvec;        // horizontal blocks of this cluster are 16-byte aligned
 
     __device__ Scanner(const ScanArgs& args, uint8_t* l) : a(args), lds(l) {
         qf = reinterpret_cast<float*>(lds + a.o_q);
@@ -312,10 +332,12 @@ struct Scanner {
         CK = reinterpret_cast<uint64_t*>(lds + a.o_ck);
         CP = reinterpret_cast<uint32_t*>(lds + a.o_cp);
         red = reinterpret_cast<uint32_t*>(lds + a.o_red);
+        taus = reinterpret_cast<dist_t*>(lds + a.o_tau);
         d = a.L.d;
         v = a.L.v;
         h = a.L.h;
         nvw = (v + 3) / 4;
+        wcnt = reinterpret_cast<uint32_t*>(taus) + (h / kHDim + (v + kHDim - 1) / kHDim + 1);
     }
 
     static __device__ __forceinline__ dist_t mask_value() {
@@ -491,6 +513,72 @@ struct Scanner {
         }
     }
 
+    // The same block distance computed by one thread: the 16 lane partial sums in registers and
+    // the same reduction tree (F32), or the exact integer sum (U8).
+    __device__ __forceinline__ dist_t horizontal_thread(uint32_t i, uint32_t hd) const {
+        if constexpr (SUMM) {
+            return sb[(size_t)(a.nw + hd / kHDim) * n + i];
+        } else if constexpr (!U8) {
+            const float* blk = reinterpret_cast<const float*>(base) + (size_t)(v + hd) * S + (size_t)i * kHDim;
+            const float* qh = qf + v + hd;
+            float p[16];
+#pragma unroll 1
+            for (uint32_t r = 0; r < 4; ++r) {  // dims 16r .. 16r+15: one step of each lane's chain
+                float x[16];
+                if (hvec) {
+#pragma unroll
+                    for (uint32_t c = 0; c < 4; ++c) {
+                        const float4 t4 = reinterpret_cast<const float4*>(blk + 16 * r)[c];
+                        x[4 * c] = t4.x;
+                        x[4 * c + 1] = t4.y;
+                        x[4 * c + 2] = t4.z;
+                        x[4 * c + 3] = t4.w;
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t l = 0; l < 16; ++l) x[l] = blk[16 * r + l];
+                }
+#pragma unroll
+                for (uint32_t l = 0; l < 16; ++l) {
+                    const float df = qh[16 * r + l] - x[l];
+                    p[l] = __builtin_fmaf(df, df, r == 0 ? 0.0f : p[l]);
+                }
+            }
+            return tree16_regs(p);
+        } else {
+            uint32_t x[16];
+            fetch_block_u8(i, hd, x);
+            return block_dist_u8(x, hd);
+        }
+    }
+
+    // U8 horizontal block of slot i as 16 packed words, and its exact distance to the query.
+    __device__ __forceinline__ void fetch_block_u8(uint32_t i, uint32_t hd, uint32_t (&x)[16]) const {
+        const uint8_t* blk = base + (size_t)(v + hd) * S + (size_t)i * kHDim;
+        if (hvec) {
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                const uint4 t4 = reinterpret_cast<const uint4*>(blk)[r];
+                x[4 * r] = t4.x;
+                x[4 * r + 1] = t4.y;
+                x[4 * r + 2] = t4.z;
+                x[4 * r + 3] = t4.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t w = 0; w < 16; ++w)
+                x[w] = static_cast<uint32_t>(blk[4 * w]) | (static_cast<uint32_t>(blk[4 * w + 1]) << 8) |
+                       (static_cast<uint32_t>(blk[4 * w + 2]) << 16) | (static_cast<uint32_t>(blk[4 * w + 3]) << 24);
+        }
+    }
+    __device__ __forceinline__ uint32_t block_dist_u8(const uint32_t (&x)[16], uint32_t hd) const {
+        const uint32_t w0 = nvw + hd / 4;
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < 16; ++w) acc += sqdist4(qw[w0 + w], x[w], qq[w0 + w]);
+        return acc;
+    }
+
     __device__ __forceinline__ uint64_t key_of(uint32_t i) const {
         return make_key(to_float(dist[i]), cid[i]);
     }
@@ -533,6 +621,19 @@ struct Scanner {
 
     // searcher.hpp:218-281
     __device__ void start(uint32_t k) {
+        if constexpr (SUMM) {
+            const uint32_t nw = a.nw, nh = h / kHDim;
+            for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+                float dd = nw ? sb[(size_t)(nw - 1) * n + i] : 0.0f;
+                for (uint32_t b = 0; b < nh; ++b) dd += sb[(size_t)(nw + b) * n + i];
+                dist[i] = dd;
+            }
+            __syncthreads();
+            bytes += (unsigned long long)n * d * 4;
+            mask_tombstones_all();
+            push_m_smallest(min(k - hl, n));
+            return;
+        }
         for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
             dist_t acc[kVT];
 #pragma unroll
@@ -632,6 +733,55 @@ struct Scanner {
             if (selection < (1.0f - 0.8f)) return 0;
         }
         const uint32_t need = static_cast<uint32_t>(ceilf(0.8f * static_cast<float>(n)));
+        if constexpr (SUMM) {
+            // every checkpoint value is at hand: count all of them in one pass, then take the
+            // first checkpoint where the loop below would stop
+            const uint32_t nw = a.nw;
+            if (nw == 0) return 0;
+            for (uint32_t t = threadIdx.x; t < nw; t += BLOCK) wcnt[t] = 0;
+            __syncthreads();
+            uint32_t c[kMaxCheckpoints];
+            dist_t tau[kMaxCheckpoints];
+            uint32_t e = 0;
+#pragma unroll
+            for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
+                c[t] = 0;
+                if (t < nw) {
+                    e = min(e + c_fetch[t], v);
+                    tau[t] = threshold(k, e);
+                }
+            }
+            for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+#pragma unroll
+                for (uint32_t t = 0; t < kMaxCheckpoints; ++t)
+                    if (t < nw) c[t] += sb[(size_t)t * n + i] >= tau[t];
+            }
+#pragma unroll
+            for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
+                if (t < nw) {
+                    uint32_t x = c[t];
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+                    if ((threadIdx.x & (kWave - 1)) == 0 && x) atomicAdd(&wcnt[t], x);
+                }
+            }
+            __syncthreads();
+            uint32_t tw = nw - 1, prev = 0;
+            e = 0;
+            for (uint32_t t = 0; t < nw; ++t) {
+                e = min(e + c_fetch[t], v);
+                bytes += (unsigned long long)n * (e - prev) * 4;
+                prev = e;
+                if (wcnt[t] >= need) {
+                    tw = t;
+                    break;
+                }
+            }
+            const float* vt = sb + (size_t)tw * n;
+            for (uint32_t i = threadIdx.x; i < n; i += BLOCK) dist[i] = vt[i];
+            __syncthreads();
+            return prev;
+        }
         uint32_t cur = 0, t = 0, npruned = 0;
         while (npruned < need && cur < v) {
             const uint32_t end = min(cur + c_fetch[t], v);
@@ -659,23 +809,7 @@ struct Scanner {
         return cur;
     }
 
-    // Stable in-place filter of pos[0..np) by dist < tau (EvaluatePruningPredicateOnPositionsArray).
-    __device__ uint32_t filter(uint32_t np, dist_t tau) {
-        uint32_t out = 0;
-        for (uint32_t j0 = 0; j0 < np; j0 += BLOCK) {
-            uint32_t j = j0 + threadIdx.x;
-            uint32_t pj = j < np ? pos[j] : 0u;
-            bool p = j < np && dist[pj] < tau;
-            uint32_t s;
-            uint32_t r = block_excl_scan<BLOCK>(p, red, &s);
-            if (p) pos[out + r] = pj;
-            out += s;
-        }
-        __syncthreads();
-        return out;
-    }
-
-    // searcher.hpp:438-540. Returns the number of survivors left in pos[].
+    // searcher.hpp:438-540. Returns the number of entries of pos[] (survivors, or ~0u once pruned).
     __device__ uint32_t prune(uint32_t k, uint32_t cur) {
         dist_t tau = threshold(k, cur);
         uint32_t np = 0;
@@ -697,49 +831,97 @@ struct Scanner {
             np += s;
         }
         __syncthreads();
-        uint32_t cv = cur, hd = 0;
-        const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16;
-        while (h && np && hd < h) {
-            for (uint32_t j = g; j < np; j += G) {
-                uint32_t i = pos[j];
-                dist_t hv = horizontal16(i, hd, l);
-                if (l == 0) dist[i] += hv;
+        // Each survivor's remaining steps depend only on its own partial distance and the step
+        // thresholds (fixed for the cluster): every thread walks its survivors through the
+        // horizontal blocks, then the vertical remainder in 64-dim chunks, and stops at the
+        // first failed test; no barrier between steps. Pruned entries of pos[] become ~0u.
+        constexpr unsigned long long elem = U8 ? 1 : 4;
+        const uint32_t nhs = h / kHDim, nvs = (v - cur + kHDim - 1) / kHDim;
+        for (uint32_t t = threadIdx.x; t < nhs + nvs; t += BLOCK)
+            taus[t] = threshold(k, t < nhs ? cur + (t + 1) * kHDim : min(cur + h + (t - nhs + 1) * kHDim, d));
+        __syncthreads();
+        const uint32_t cur0 = cur;
+        uint32_t alive_cnt = 0;
+        for (uint32_t j = threadIdx.x; j < np; j += BLOCK) {
+            const uint32_t i = pos[j];
+            dist_t x = dist[i];
+            bool alive = true;
+            uint32_t t = 0;
+            if constexpr (SUMM) {
+                // stored block distances: 16 loads in flight ahead of the tests
+                for (uint32_t b0 = 0; b0 < nhs && alive; b0 += 16) {
+                    float hv[16];
+#pragma unroll
+                    for (uint32_t u = 0; u < 16; ++u)
+                        hv[u] = b0 + u < nhs ? sb[(size_t)(a.nw + b0 + u) * n + i] : 0.0f;
+#pragma unroll
+                    for (uint32_t u = 0; u < 16; ++u) {
+                        if (b0 + u < nhs && alive) {
+                            x += hv[u];
+                            bytes_local += kHDim * elem;
+                            alive = x < taus[t];
+                            ++t;
+                        }
+                    }
+                }
+            } else if constexpr (U8) {
+                // the next block is in flight while the current one is tested
+                uint32_t nx[16];
+                if (nhs) fetch_block_u8(i, 0, nx);
+#pragma unroll 1
+                for (uint32_t b = 0; b < nhs; ++b) {
+                    uint32_t cx[16];
+#pragma unroll
+                    for (uint32_t w = 0; w < 16; ++w) cx[w] = nx[w];
+                    if (b + 1 < nhs) fetch_block_u8(i, (b + 1) * kHDim, nx);
+                    x += block_dist_u8(cx, b * kHDim);
+                    bytes_local += kHDim * elem;
+                    if (!(x < taus[t])) {
+                        alive = false;
+                        break;
+                    }
+                    ++t;
+                }
+            } else {
+                for (uint32_t hd = 0; hd < h; hd += kHDim, ++t) {
+                    x += horizontal_thread(i, hd);
+                    bytes_local += kHDim * elem;
+                    if (!(x < taus[t])) {
+                        alive = false;
+                        break;
+                    }
+                }
             }
-            __syncthreads();
-            bytes += (unsigned long long)np * kHDim * (U8 ? 1 : 4);
-            hd += kHDim;
-            cur += kHDim;
-            tau = threshold(k, cur);
-            np = filter(np, tau);
-        }
-        while (np && cv < v) {
-            const uint32_t end = min(cv + kHDim, v);
-            for (uint32_t j = threadIdx.x; j < np; j += BLOCK) {
-                uint32_t i = pos[j];
-                dist[i] = vertical_one(dist[i], i, cv, end);
+            if (alive) {
+                for (uint32_t cv = cur0; cv < v; cv += kHDim, ++t) {
+                    const uint32_t end = min(cv + kHDim, v);
+                    x = vertical_one(x, i, cv, end);
+                    bytes_local += (end - cv) * elem;
+                    if (!(x < taus[t])) {
+                        alive = false;
+                        break;
+                    }
+                }
             }
-            __syncthreads();
-            bytes += (unsigned long long)np * (end - cv) * (U8 ? 1 : 4);
-            cur = min(cur + kHDim, d);
-            cv = end;
-            tau = threshold(k, cur);
-            np = filter(np, tau);
-            if (cur == d) break;
+            dist[i] = x;
+            if (alive) ++alive_cnt;
+            else pos[j] = ~0u;
         }
+        bytes_local += 4ull * alive_cnt;  // Merge reads one distance per survivor
+        __syncthreads();
         return np;
     }
 
-    // searcher.hpp:542-566
+    // searcher.hpp:542-566, over the entries of pos[] that survived the prune
     __device__ void merge(uint32_t k, uint32_t np) {
         for (uint32_t j0 = 0; j0 < np; j0 += BLOCK) {
             uint32_t j = j0 + threadIdx.x;
-            bool valid = j < np;
-            uint32_t i = valid ? pos[j] : 0u;
+            uint32_t i = j < np ? pos[j] : ~0u;
+            bool valid = i != ~0u;
             uint64_t key = valid ? key_of(i) : kKeyInf;
             uint32_t pay = valid ? pay_of(i) : 0u;
             hl = list_insert_chunk<BLOCK>(valid, key, pay, H, HP, hl, k, CK, CP, O, OP, red);
         }
-        bytes += 4ull * np;
     }
 
     __device__ void run(uint32_t q) {
@@ -763,6 +945,8 @@ struct Scanner {
             cid = a.L.ids + a.L.slot_off[c];
             ntomb = a.L.n_tomb[c];
             tbm = a.L.tomb + a.L.tomb_off[c];
+            hvec = ((size_t)v * S * (U8 ? 1 : 4)) % 16 == 0;
+            if constexpr (SUMM) sb = a.summ + ((size_t)q * a.region + a.soff[(size_t)q * a.order_stride + ci]) * a.ns;
             if (hl < k) {
                 if constexpr (FILT) filtered_start(k, passing);
                 else start(k);
@@ -771,6 +955,16 @@ struct Scanner {
             uint32_t cur = warmup(k, passing);
             uint32_t np = prune(k, cur);
             if (np) merge(k, np);
+        }
+        // this thread's share of the algorithmic bytes, summed over the workgroup
+        {
+            unsigned long long x = bytes_local;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            __syncthreads();
+            if ((threadIdx.x & (kWave - 1)) == 0) CK[threadIdx.x / kWave] = x;
+            __syncthreads();
+            for (uint32_t w = 0; w < BLOCK / kWave; ++w) bytes += CK[w];
         }
         // searcher.hpp:568-585
         uint32_t cnt = hl;
@@ -789,11 +983,11 @@ struct Scanner {
     }
 };
 
-template <bool U8, bool FILT, uint32_t BLOCK>
+template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false>
 __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     if (blockIdx.x >= a.nq) return;
-    Scanner<U8, FILT, BLOCK> s(a, lds_raw);
+    Scanner<U8, FILT, BLOCK, SUMM> s(a, lds_raw);
     s.run(blockIdx.x);
 }
 

@@ -1,0 +1,230 @@
+// Two-phase meso-level scan of two-level indexes.
+//
+// The meso level is searched with k = nprobe over about half of the meso clusters per query
+// (index.hpp:545-566); with k that large ADSampling prunes little, so the pruned scan streams
+// most of every visited meso cluster once per query. The two phases below compute the same
+// results with the reads shared across queries:
+//
+//   phase 1  invert the per-query visit lists into per-cluster lists of (query, rank) entries and
+//            compute, for every visited (query, vector) pair, every partial distance the scan can
+//            compare: the vertical fma chain at each warmup checkpoint (the cumulative
+//            DIMENSIONS_FETCHING_SIZES clipped to v, common.hpp:47-49) and each 64-dim horizontal
+//            block distance in the 16-lane order. A work item is (cluster, 8 entries) on one
+//            wave; the 8 queries are wave-uniform and read through the scalar cache, the cluster's
+//            vectors map one per lane (coalesced vertical reads).
+//   phase 2  k_scan<..., SUMM = true>: the unchanged Start / Warmup / Prune / Merge control flow
+//            per query, reading those values instead of the vectors.
+//
+// Every value the replay compares is produced by the same float operations in the same order as
+// in the pruned scan, so thresholds, survivors, heap contents and byte counts are identical.
+#pragma once
+#include "pdx_kernels.hpp"
+
+namespace pdxg {
+
+constexpr uint32_t kMesoQ = 8;      // queries per work item (one wave)
+constexpr uint32_t kMesoWaves = 4;  // work items per workgroup
+
+struct MesoArgs {
+    LevelDev L;                 // the meso level (F32)
+    const float* qf;            // rotated queries [nq x d]
+    const uint32_t* order;      // meso visit order [nq x stride]
+    const uint32_t* nvisit;     // meso clusters visited per query
+    uint32_t stride, nq, nv_max;
+    uint32_t* cnt;              // [nc] entries per cluster
+    uint32_t* cur;              // [nc] scatter cursors
+    uint32_t* lstart;           // [nc + 1] first entry of each cluster
+    uint32_t* cstart;           // [nc + 1] first work item of each cluster
+    uint32_t* list;             // entries q * stride + rank, grouped by cluster
+    uint32_t* soff;             // [nq x stride] first summarised vector of (query, rank) in its region
+    float* summ;                // per query: `region` vectors x ns floats
+    unsigned long long region;
+    uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector
+    uint32_t wg_per_xcd;        // summary grid = 8 x wg_per_xcd workgroups
+};
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_meso_count(MesoArgs a) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)a.nq * a.nv_max) return;
+    const uint32_t q = static_cast<uint32_t>(t / a.nv_max), r = static_cast<uint32_t>(t % a.nv_max);
+    if (r < a.nvisit[q]) atomicAdd(&a.cnt[a.order[(size_t)q * a.stride + r]], 1u);
+}
+
+// Entry and work-item offsets per cluster (one workgroup of 1024).
+__global__ __launch_bounds__(1024) void k_meso_offsets(MesoArgs a) {
+    __shared__ uint32_t red[1024 / kWave + 1];
+    const uint32_t nc = a.L.nc;
+    uint32_t lbase = 0, cbase = 0;
+    for (uint32_t m0 = 0; m0 < nc; m0 += 1024) {
+        const uint32_t m = m0 + threadIdx.x;
+        const uint32_t c = m < nc ? a.cnt[m] : 0u;
+        uint32_t lt, ct;
+        const uint32_t lx = block_excl_sum<1024>(c, red, &lt);
+        const uint32_t cx = block_excl_sum<1024>((c + kMesoQ - 1) / kMesoQ, red, &ct);
+        if (m < nc) {
+            a.lstart[m] = lbase + lx;
+            a.cstart[m] = cbase + cx;
+            a.cur[m] = 0;
+        }
+        lbase += lt;
+        cbase += ct;
+    }
+    if (threadIdx.x == 0) {
+        a.lstart[nc] = lbase;
+        a.cstart[nc] = cbase;
+    }
+}
+
+__global__ void k_meso_scatter(MesoArgs a) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)a.nq * a.nv_max) return;
+    const uint32_t q = static_cast<uint32_t>(t / a.nv_max), r = static_cast<uint32_t>(t % a.nv_max);
+    if (r >= a.nvisit[q]) return;
+    const uint32_t m = a.order[(size_t)q * a.stride + r];
+    a.list[a.lstart[m] + atomicAdd(&a.cur[m], 1u)] = q * a.stride + r;
+}
+
+// Summary offsets of a query's visits: running sum of the visited clusters' sizes.
+__global__ __launch_bounds__(256) void k_meso_soff(MesoArgs a) {
+    __shared__ uint32_t red[256 / kWave + 1];
+    const uint32_t q = blockIdx.x;
+    const uint32_t nv = a.nvisit[q];
+    uint32_t base = 0;
+    for (uint32_t r0 = 0; r0 < nv; r0 += 256) {
+        const uint32_t r = r0 + threadIdx.x;
+        const uint32_t x = r < nv ? a.L.n_used[a.order[(size_t)q * a.stride + r]] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_sum<256>(x, red, &tot);
+        if (r < nv) a.soff[(size_t)q * a.stride + r] = base + ex;
+        base += tot;
+    }
+}
+
+// Query rows are read with wave-uniform addresses through the scalar cache.
+typedef const __attribute__((address_space(4))) float const_float;
+__device__ __forceinline__ const_float* to_const(const float* p) {
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wold-style-cast"
+    return (const_float*)p;
+#pragma clang diagnostic pop
+}
+
+// Phase 1. Summary block of (query, rank), component-major: value t of vector i at
+// [t * n + i]; t < nw: vertical chain over dims [0, end_t); t = nw + b: horizontal block b.
+__global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
+    const uint32_t wg = (blockIdx.x % 8) * a.wg_per_xcd + blockIdx.x / 8;  // items of an XCD are contiguous
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t item = wg * kMesoWaves + w;
+    const uint32_t nc = a.L.nc;
+    if (item >= a.cstart[nc]) return;  // wave-uniform; the kernel has no barriers
+    uint32_t lo = 0, hi = nc;          // cstart[lo] <= item < cstart[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.cstart[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t m = lo;
+    const uint32_t e0 = a.lstart[m] + (item - a.cstart[m]) * kMesoQ;
+    const uint32_t ne = min(kMesoQ, a.lstart[m + 1] - e0);
+    const uint32_t n = a.L.n_used[m];
+    if (n == 0) return;
+    const uint32_t d = a.L.d, v = a.L.v, nh = a.L.h / kHDim, S = a.L.cap[m];
+    const float* data = reinterpret_cast<const float*>(a.L.data + a.L.data_off[m]);
+    const bool aligned = ((size_t)v * S) % 4 == 0;  // 16-byte aligned horizontal blocks
+
+    // per query: row offset of the rotated query and first summarised vector of its block
+    // (region index q * region + soff < 2^32, checked by the host); ~0u = not stored
+    const_float* qbase = to_const(a.qf);
+    uint32_t qo[kMesoQ], ob[kMesoQ];
+#pragma unroll
+    for (uint32_t u = 0; u < kMesoQ; ++u) {
+        const bool ok = u < ne;
+        const uint32_t e = __builtin_amdgcn_readfirstlane(a.list[e0 + (ok ? u : 0u)]);
+        const uint32_t q = e / a.stride;
+        const uint32_t so = __builtin_amdgcn_readfirstlane(a.soff[e]);
+        qo[u] = q * d;
+        ob[u] = (ok && (unsigned long long)so + n <= a.region) ? static_cast<uint32_t>(q * a.region + so) : ~0u;
+    }
+
+    for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        const bool valid = i < n;
+        const uint32_t ii = valid ? i : n - 1;
+        const float* col = data + ii;
+        float acc[kMesoQ];
+#pragma unroll
+        for (uint32_t u = 0; u < kMesoQ; ++u) acc[u] = 0.0f;
+        uint32_t j = 0, end = 0;
+        for (uint32_t t = 0; t < a.nw; ++t) {
+            end = min(end + c_fetch[t], v);
+            for (; j + 4 <= end; j += 4) {
+                float x[4];
+#pragma unroll
+                for (uint32_t s = 0; s < 4; ++s) x[s] = col[(size_t)(j + s) * S];
+#pragma unroll
+                for (uint32_t u = 0; u < kMesoQ; ++u)
+#pragma unroll
+                    for (uint32_t s = 0; s < 4; ++s) {
+                        const float df = qbase[qo[u] + j + s] - x[s];
+                        acc[u] = __builtin_fmaf(df, df, acc[u]);
+                    }
+            }
+            for (; j < end; ++j) {
+                const float x = col[(size_t)j * S];
+#pragma unroll
+                for (uint32_t u = 0; u < kMesoQ; ++u) {
+                    const float df = qbase[qo[u] + j] - x;
+                    acc[u] = __builtin_fmaf(df, df, acc[u]);
+                }
+            }
+            if (valid) {
+#pragma unroll
+                for (uint32_t u = 0; u < kMesoQ; ++u)
+                    if (ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)t * n + i] = acc[u];
+            }
+        }
+        for (uint32_t b = 0; b < nh; ++b) {
+            const float* blk = data + (size_t)(v + b * kHDim) * S + (size_t)ii * kHDim;
+            float xb[kHDim];
+            if (aligned) {
+#pragma unroll
+                for (uint32_t r = 0; r < kHDim / 4; ++r) {
+                    const float4 t4 = reinterpret_cast<const float4*>(blk)[r];
+                    xb[4 * r] = t4.x;
+                    xb[4 * r + 1] = t4.y;
+                    xb[4 * r + 2] = t4.z;
+                    xb[4 * r + 3] = t4.w;
+                }
+            } else {
+#pragma unroll
+                for (uint32_t l = 0; l < kHDim; ++l) xb[l] = blk[l];
+            }
+            for (uint32_t u = 0; u < kMesoQ; ++u) {
+                const_float* qh = qbase + qo[u] + v + b * kHDim;
+                float p[16];
+#pragma unroll
+                for (uint32_t l = 0; l < 16; ++l) {
+                    const float df = qh[l] - xb[l];
+                    p[l] = __builtin_fmaf(df, df, 0.0f);
+                }
+#pragma unroll
+                for (uint32_t r = 1; r < 4; ++r)
+#pragma unroll
+                    for (uint32_t l = 0; l < 16; ++l) {
+                        const float df = qh[16 * r + l] - xb[16 * r + l];
+                        p[l] = __builtin_fmaf(df, df, p[l]);
+                    }
+                const float hv = tree16_regs(p);
+                if (valid && ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)(a.nw + b) * n + i] = hv;
+            }
+        }
+    }
+}
+
+}  // namespace pdxg

@@ -128,3 +128,51 @@ def test_every_scan_block_size_gives_the_oracle_results(gpu_lib, kind):
     assert plan[1] in (64, 128, 256, 512, 1024)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(ids.cpu().numpy().view(np.uint32), oi)
+
+
+@pytest.mark.parametrize("kind,n,d,nlist,n_meso,nq", [("pdx_tree_u8", 4000, 768, 40, 0, 24),
+                                                      ("pdx_tree_f32", 3000, 128, 24, 0, 24),
+                                                      ("pdx_tree_u8", 2000, 130, 16, 0, 24),
+                                                      ("pdx_tree_f32", 20000, 256, 400, 4, 150),
+                                                      ("pdx_tree_u8", 12000, 1024, 300, 3, 70)])
+def test_two_phase_meso_scan_matches_pruned_and_oracle(gpu_lib, kind, n, d, nlist, n_meso, nq):
+    """The meso level through summaries + replay and through the pruned scan: both equal the oracle.
+
+    The last two cases have meso clusters of 75-100 members (several 64-lane chunks) visited by
+    many queries (several 8-query work items per cluster)."""
+    import torch
+
+    path, x, q0 = index_file(kind, n, d, nlist, n_meso=n_meso)
+    rng = np.random.default_rng(11)
+    q = np.concatenate([q0, x[rng.integers(0, n, max(0, nq - len(q0)))] +
+                        np.float32(0.05) * rng.standard_normal((max(0, nq - len(q0)), d), dtype=np.float32)])[:nq]
+    q = np.ascontiguousarray(q, np.float32)
+    o = OracleIndex(path)
+    g = gpu_lib.GpuIndex(pf.load(path))
+    K = 20
+    npb = rng.integers(1, nlist + 1, nq).astype(np.uint32)   # halved and full meso visit counts
+    kq = rng.choice([1, 5, 20], nq).astype(np.uint32)
+    oi, od, oid, oc, ob = o.search(q, kq, npb)
+    dev = torch.device("cuda", 0)
+    qd = torch.from_numpy(q).to(dev)
+    kd = torch.from_numpy(kq.view(np.int32)).to(dev)
+    nd = torch.from_numpy(npb.view(np.int32)).to(dev)
+    ws = gpu_lib.Workspace(g, nq, K, nlist)
+    sp = torch.cuda.current_stream().cuda_stream
+    for summ in (True, False):
+        ws.set_meso_summary(summ)
+        assert ws.meso_summary == summ
+        for b in (64, 256, 1024):
+            print(f"{kind} d={d} summary={summ} block {b}", flush=True)
+            ws.set_plan(b, 0)
+            ids = torch.zeros((nq, K), dtype=torch.int32, device=dev)
+            dd = torch.zeros((nq, K), dtype=torch.float32, device=dev)
+            idd = torch.zeros((nq, K), dtype=torch.int32, device=dev)
+            cnt = torch.zeros(nq, dtype=torch.int32, device=dev)
+            nb = torch.zeros(nq, dtype=torch.int64, device=dev)
+            ws.search(qd.data_ptr(), nq, kd.data_ptr(), nd.data_ptr(), K, nlist, K, ids.data_ptr(), dd.data_ptr(),
+                      idd.data_ptr(), cnt.data_ptr(), nb.data_ptr(), stream=sp)
+            torch.cuda.synchronize()
+            res = (ids.cpu().numpy().view(np.uint32), dd.cpu().numpy(), idd.cpu().numpy().view(np.uint32),
+                   cnt.cpu().numpy().view(np.uint32), nb.cpu().numpy().view(np.uint64))
+            _compare((oi, od, oid, oc, ob), res, o.is_u8)
