@@ -234,7 +234,7 @@ struct Level {
 
 struct LdsPlan {
     uint32_t bytes = 0;
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau;
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_rat, o_ring;
 };
 
 LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
@@ -261,7 +261,9 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
     p.o_op = take(kmax * 4);
     p.o_cp = take(block * 4);
     p.o_red = take(64);
-    p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + kMaxCheckpoints) * 4);
+    p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + kMaxCheckpoints + 1) * 4);
+    p.o_rat = take(L.d * 4);
+    p.o_ring = take(kRing * sizeof(ClusterMeta));
     p.bytes = (off + 15) / 16 * 16;
     return p;
 }
@@ -446,6 +448,8 @@ void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_
     a.o_cp = p.o_cp;
     a.o_red = p.o_red;
     a.o_tau = p.o_tau;
+    a.o_rat = p.o_rat;
+    a.o_ring = p.o_ring;
     auto kern = k_scan<U8, FILT, BLOCK, SUMM>;
     if (p.bytes > 65536)
         HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -62,8 +62,16 @@ struct ScanArgs {
     uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector (nw + h/64)
     uint32_t nq;
     // dynamic LDS layout (byte offsets)
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau;
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_rat, o_ring;
 };
+
+// Per-cluster parameters of the next kRing visited clusters, staged in LDS in one batch so that
+// a cluster's scan starts without a dependent global round trip.
+struct ClusterMeta {
+    unsigned long long data_off, slot_off, tomb_off;
+    uint32_t c, n_live, n_used, cap, n_tomb, aux;  // aux: passing slots (filtered) / summary offset
+};
+constexpr uint32_t kRing = 64;
 
 // Largest number of warmup checkpoints (entries of DIMENSIONS_FETCHING_SIZES used).
 constexpr uint32_t kMaxCheckpoints = 20;
@@ -302,6 +310,8 @@ struct Scanner {
     uint64_t *H, *T, *O, *CK;
     uint32_t *HP, *TP, *OP, *CP, *red;
     dist_t* taus;     // thresholds of the current prune steps
+    float* rat;       // ADSampling ratio table [d]
+    ClusterMeta* ring;
     uint32_t* wcnt;   // SUMM: pruned counts per warmup checkpoint
     uint32_t d, v, h, nvw;
     uint32_t hl = 0;
@@ -333,6 +343,8 @@ struct Scanner {
         CP = reinterpret_cast<uint32_t*>(lds + a.o_cp);
         red = reinterpret_cast<uint32_t*>(lds + a.o_red);
         taus = reinterpret_cast<dist_t*>(lds + a.o_tau);
+        rat = reinterpret_cast<float*>(lds + a.o_rat);
+        ring = reinterpret_cast<ClusterMeta*>(lds + a.o_ring);
         d = a.L.d;
         v = a.L.v;
         h = a.L.h;
@@ -357,7 +369,7 @@ struct Scanner {
     // GetPruningThreshold: adsampling.hpp:91-98 with the U8 conversion of searcher.hpp:75-91.
     __device__ __forceinline__ dist_t threshold(uint32_t k, uint32_t cur) const {
         float top = key_dist(H[k - 1]);
-        float ratio = cur == d ? 1.0f : a.ratios[cur];
+        float ratio = cur == d ? 1.0f : rat[cur];
         float tf = top * ratio;
         if constexpr (U8) {
             float sc = tf * a.L.scale_sq;
@@ -370,6 +382,7 @@ struct Scanner {
     __device__ __forceinline__ uint32_t qbyte(uint32_t j) const { return (qw[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
 
     __device__ void load_query(uint32_t q) {
+        for (uint32_t j = threadIdx.x; j < d; j += BLOCK) rat[j] = a.ratios[j];
         if constexpr (!U8) {
             for (uint32_t j = threadIdx.x; j < d; j += BLOCK) qf[j] = a.qf[(size_t)q * d + j];
         } else {
@@ -809,8 +822,68 @@ struct Scanner {
         return cur;
     }
 
-    // searcher.hpp:438-540. Returns the number of entries of pos[] (survivors, or ~0u once pruned).
-    __device__ uint32_t prune(uint32_t k, uint32_t cur) {
+    // Remaining steps of one survivor i (partial distance x over the warmup dims): horizontal
+    // blocks, then the vertical remainder in 64-dim chunks, each followed by its threshold test
+    // (taus[]); stops at the first failed test. Returns whether i survives; x is its distance.
+    __device__ __forceinline__ bool walk(uint32_t i, dist_t& x, uint32_t cur0) {
+        constexpr unsigned long long elem = U8 ? 1 : 4;
+        const uint32_t nhs = h / kHDim;
+        bool alive = true;
+        uint32_t t = 0;
+        if constexpr (SUMM) {
+            // stored block distances: 16 loads in flight ahead of the tests
+            for (uint32_t b0 = 0; b0 < nhs && alive; b0 += 16) {
+                float hv[16];
+#pragma unroll
+                for (uint32_t u = 0; u < 16; ++u) hv[u] = b0 + u < nhs ? sb[(size_t)(a.nw + b0 + u) * n + i] : 0.0f;
+#pragma unroll
+                for (uint32_t u = 0; u < 16; ++u) {
+                    if (b0 + u < nhs && alive) {
+                        x += hv[u];
+                        bytes_local += kHDim * elem;
+                        alive = x < taus[t];
+                        ++t;
+                    }
+                }
+            }
+        } else if constexpr (U8) {
+            // the next block is in flight while the current one is tested
+            uint32_t nx[16];
+            if (nhs) fetch_block_u8(i, 0, nx);
+#pragma unroll 1
+            for (uint32_t b = 0; b < nhs; ++b) {
+                uint32_t cx[16];
+#pragma unroll
+                for (uint32_t w = 0; w < 16; ++w) cx[w] = nx[w];
+                if (b + 1 < nhs) fetch_block_u8(i, (b + 1) * kHDim, nx);
+                x += block_dist_u8(cx, b * kHDim);
+                bytes_local += kHDim * elem;
+                if (!(x < taus[t])) return false;
+                ++t;
+            }
+        } else {
+            for (uint32_t hd = 0; hd < h; hd += kHDim, ++t) {
+                x += horizontal_thread(i, hd);
+                bytes_local += kHDim * elem;
+                if (!(x < taus[t])) return false;
+            }
+        }
+        if (!alive) return false;
+        for (uint32_t cv = cur0; cv < v; cv += kHDim, ++t) {
+            const uint32_t end = min(cv + kHDim, v);
+            x = vertical_one(x, i, cv, end);
+            bytes_local += (end - cv) * elem;
+            if (!(x < taus[t])) return false;
+        }
+        return true;
+    }
+
+    // Prune and Merge (searcher.hpp:438-566). The survivors of the test at `cur` are compacted
+    // once; each survivor's remaining steps depend only on its own partial distance and the step
+    // thresholds (fixed for the cluster), so every thread walks its survivors to the end with no
+    // barrier between steps. The survivors' keys stay in registers (kR per thread) for the merge.
+    static constexpr uint32_t kR = 4;
+    __device__ void prune_merge(uint32_t k, uint32_t cur) {
         dist_t tau = threshold(k, cur);
         uint32_t np = 0;
         for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
@@ -830,97 +903,52 @@ struct Scanner {
             if (p) pos[np + r] = i;
             np += s;
         }
-        __syncthreads();
-        // Each survivor's remaining steps depend only on its own partial distance and the step
-        // thresholds (fixed for the cluster): every thread walks its survivors through the
-        // horizontal blocks, then the vertical remainder in 64-dim chunks, and stops at the
-        // first failed test; no barrier between steps. Pruned entries of pos[] become ~0u.
-        constexpr unsigned long long elem = U8 ? 1 : 4;
+        if (np == 0) return;
         const uint32_t nhs = h / kHDim, nvs = (v - cur + kHDim - 1) / kHDim;
         for (uint32_t t = threadIdx.x; t < nhs + nvs; t += BLOCK)
             taus[t] = threshold(k, t < nhs ? cur + (t + 1) * kHDim : min(cur + h + (t - nhs + 1) * kHDim, d));
         __syncthreads();
-        const uint32_t cur0 = cur;
+        uint64_t key[kR];
+        uint32_t pay[kR];
         uint32_t alive_cnt = 0;
-        for (uint32_t j = threadIdx.x; j < np; j += BLOCK) {
+#pragma unroll
+        for (uint32_t r = 0; r < kR; ++r) {
+            key[r] = kKeyInf;
+            pay[r] = 0;
+            const uint32_t j = threadIdx.x + r * BLOCK;
+            if (j < np) {
+                const uint32_t i = pos[j];
+                const uint32_t id = cid[i];
+                dist_t x = dist[i];
+                if (walk(i, x, cur)) {
+                    key[r] = make_key(to_float(x), id);
+                    if constexpr (U8) pay[r] = x;
+                    ++alive_cnt;
+                }
+            }
+        }
+        for (uint32_t j = threadIdx.x + kR * BLOCK; j < np; j += BLOCK) {  // beyond the register slots
             const uint32_t i = pos[j];
             dist_t x = dist[i];
-            bool alive = true;
-            uint32_t t = 0;
-            if constexpr (SUMM) {
-                // stored block distances: 16 loads in flight ahead of the tests
-                for (uint32_t b0 = 0; b0 < nhs && alive; b0 += 16) {
-                    float hv[16];
-#pragma unroll
-                    for (uint32_t u = 0; u < 16; ++u)
-                        hv[u] = b0 + u < nhs ? sb[(size_t)(a.nw + b0 + u) * n + i] : 0.0f;
-#pragma unroll
-                    for (uint32_t u = 0; u < 16; ++u) {
-                        if (b0 + u < nhs && alive) {
-                            x += hv[u];
-                            bytes_local += kHDim * elem;
-                            alive = x < taus[t];
-                            ++t;
-                        }
-                    }
-                }
-            } else if constexpr (U8) {
-                // the next block is in flight while the current one is tested
-                uint32_t nx[16];
-                if (nhs) fetch_block_u8(i, 0, nx);
-#pragma unroll 1
-                for (uint32_t b = 0; b < nhs; ++b) {
-                    uint32_t cx[16];
-#pragma unroll
-                    for (uint32_t w = 0; w < 16; ++w) cx[w] = nx[w];
-                    if (b + 1 < nhs) fetch_block_u8(i, (b + 1) * kHDim, nx);
-                    x += block_dist_u8(cx, b * kHDim);
-                    bytes_local += kHDim * elem;
-                    if (!(x < taus[t])) {
-                        alive = false;
-                        break;
-                    }
-                    ++t;
-                }
+            if (walk(i, x, cur)) {
+                dist[i] = x;
+                ++alive_cnt;
             } else {
-                for (uint32_t hd = 0; hd < h; hd += kHDim, ++t) {
-                    x += horizontal_thread(i, hd);
-                    bytes_local += kHDim * elem;
-                    if (!(x < taus[t])) {
-                        alive = false;
-                        break;
-                    }
-                }
+                pos[j] = ~0u;
             }
-            if (alive) {
-                for (uint32_t cv = cur0; cv < v; cv += kHDim, ++t) {
-                    const uint32_t end = min(cv + kHDim, v);
-                    x = vertical_one(x, i, cv, end);
-                    bytes_local += (end - cv) * elem;
-                    if (!(x < taus[t])) {
-                        alive = false;
-                        break;
-                    }
-                }
-            }
-            dist[i] = x;
-            if (alive) ++alive_cnt;
-            else pos[j] = ~0u;
         }
         bytes_local += 4ull * alive_cnt;  // Merge reads one distance per survivor
         __syncthreads();
-        return np;
-    }
-
-    // searcher.hpp:542-566, over the entries of pos[] that survived the prune
-    __device__ void merge(uint32_t k, uint32_t np) {
-        for (uint32_t j0 = 0; j0 < np; j0 += BLOCK) {
-            uint32_t j = j0 + threadIdx.x;
-            uint32_t i = j < np ? pos[j] : ~0u;
-            bool valid = i != ~0u;
-            uint64_t key = valid ? key_of(i) : kKeyInf;
-            uint32_t pay = valid ? pay_of(i) : 0u;
-            hl = list_insert_chunk<BLOCK>(valid, key, pay, H, HP, hl, k, CK, CP, O, OP, red);
+#pragma unroll
+        for (uint32_t r = 0; r < kR; ++r)
+            if (r * BLOCK < np)
+                hl = list_insert_chunk<BLOCK>(key[r] != kKeyInf, key[r], pay[r], H, HP, hl, k, CK, CP, O, OP, red);
+        for (uint32_t j0 = kR * BLOCK; j0 < np; j0 += BLOCK) {
+            const uint32_t j = j0 + threadIdx.x;
+            const uint32_t i = j < np ? pos[j] : ~0u;
+            const bool valid = i != ~0u;
+            hl = list_insert_chunk<BLOCK>(valid, valid ? key_of(i) : kKeyInf, valid ? pay_of(i) : 0u, H, HP, hl, k, CK,
+                                          CP, O, OP, red);
         }
     }
 
@@ -931,30 +959,49 @@ struct Scanner {
         if (a.order_count) nvis = min(nvis, a.order_count[q]);
         const uint32_t* ord = a.order + (size_t)q * a.order_stride;
         for (uint32_t ci = 0; ci < nvis; ++ci) {
-            const uint32_t c = ord[ci];
+            if (ci % kRing == 0) {  // stage the parameters of the next kRing clusters
+                __syncthreads();
+                const uint32_t cnt = min(kRing, nvis - ci);
+                for (uint32_t t = threadIdx.x; t < cnt; t += BLOCK) {
+                    ClusterMeta mm;
+                    const uint32_t c = ord[ci + t];
+                    mm.c = c;
+                    mm.n_live = a.L.n_live[c];
+                    mm.n_used = a.L.n_used[c];
+                    mm.cap = a.L.cap[c];
+                    mm.n_tomb = a.L.n_tomb[c];
+                    mm.data_off = a.L.data_off[c];
+                    mm.slot_off = a.L.slot_off[c];
+                    mm.tomb_off = a.L.tomb_off[c];
+                    mm.aux = 0;
+                    if constexpr (FILT) mm.aux = a.n_pass[c];
+                    if constexpr (SUMM) mm.aux = a.soff[(size_t)q * a.order_stride + ci + t];
+                    ring[t] = mm;
+                }
+                __syncthreads();
+            }
+            const ClusterMeta& mm = ring[ci % kRing];
             uint32_t passing = 0;
             if constexpr (FILT) {
-                passing = a.n_pass[c];
+                passing = mm.aux;
                 if (passing == 0) continue;
-                csel = a.sel + a.L.slot_off[c];
+                csel = a.sel + mm.slot_off;
             }
-            if (a.L.n_live[c] == 0) continue;
-            n = a.L.n_used[c];
-            S = a.L.cap[c];
-            base = a.L.data + a.L.data_off[c];
-            cid = a.L.ids + a.L.slot_off[c];
-            ntomb = a.L.n_tomb[c];
-            tbm = a.L.tomb + a.L.tomb_off[c];
+            if (mm.n_live == 0) continue;
+            n = mm.n_used;
+            S = mm.cap;
+            base = a.L.data + mm.data_off;
+            cid = a.L.ids + mm.slot_off;
+            ntomb = mm.n_tomb;
+            tbm = a.L.tomb + mm.tomb_off;
             hvec = ((size_t)v * S * (U8 ? 1 : 4)) % 16 == 0;
-            if constexpr (SUMM) sb = a.summ + ((size_t)q * a.region + a.soff[(size_t)q * a.order_stride + ci]) * a.ns;
+            if constexpr (SUMM) sb = a.summ + ((size_t)q * a.region + mm.aux) * a.ns;
             if (hl < k) {
                 if constexpr (FILT) filtered_start(k, passing);
                 else start(k);
                 continue;
             }
-            uint32_t cur = warmup(k, passing);
-            uint32_t np = prune(k, cur);
-            if (np) merge(k, np);
+            prune_merge(k, warmup(k, passing));
         }
         // this thread's share of the algorithmic bytes, summed over the workgroup
         {
