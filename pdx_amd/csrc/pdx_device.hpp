@@ -92,6 +92,21 @@ __device__ __forceinline__ uint32_t block_count(bool p, uint32_t* s_red) {
     return tot;
 }
 
+// Block-wide sum of a wave-uniform value (each wave contributes its value once).
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint32_t block_sum_uniform(uint32_t wave_value, uint32_t* s_red) {
+    constexpr uint32_t NW = BLOCK / kWave;
+    if constexpr (NW == 1) return wave_value;
+    const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) s_red[w] = wave_value;
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < NW; ++i) tot += s_red[i];
+    __syncthreads();
+    return tot;
+}
+
 // Exclusive prefix count of p in thread order; *total receives the block-wide count.
 template <uint32_t BLOCK>
 __device__ __forceinline__ uint32_t block_excl_scan(bool p, uint32_t* s_red, uint32_t* total) {

@@ -415,6 +415,23 @@ struct Scanner {
         if constexpr (!U8) {
             const float* data = reinterpret_cast<const float*>(base);
             uint32_t j = da;
+            // 8 dims x kVT vectors of loads in flight per round trip
+            for (; j + 8 <= db; j += 8) {
+                float x[8][kVT];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u)
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const float qj = qf[j + u];
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t) {
+                        float df = qj - x[u][t];
+                        acc[t] = __builtin_fmaf(df, df, acc[t]);
+                    }
+                }
+            }
             for (; j + 4 <= db; j += 4) {
                 float x[4][kVT];
 #pragma unroll
@@ -441,6 +458,22 @@ struct Scanner {
             }
         } else {
             uint32_t j = da;
+            // 32 dims (8 words) x kVT vectors of loads in flight per round trip
+            for (; j + 32 <= db; j += 32) {
+                uint32_t x[8][kVT];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u)
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t)
+                        x[u][t] = *reinterpret_cast<const uint32_t*>(base + (size_t)(j + 4 * u) * S + 4 * idx[t]);
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const uint32_t w = (j >> 2) + u;
+                    const uint32_t qv = qw[w], qs = qq[w];
+#pragma unroll
+                    for (uint32_t t = 0; t < kVT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
+                }
+            }
             for (; j + 16 <= db; j += 16) {
                 uint32_t x[4][kVT];
 #pragma unroll
@@ -535,6 +568,33 @@ struct Scanner {
             const float* blk = reinterpret_cast<const float*>(base) + (size_t)(v + hd) * S + (size_t)i * kHDim;
             const float* qh = qf + v + hd;
             float p[16];
+            if constexpr (BLOCK < 1024) {  // half a block in flight per round trip (32 registers)
+#pragma unroll 1
+                for (uint32_t hb = 0; hb < 2; ++hb) {
+                    float x[32];
+                    if (hvec) {
+#pragma unroll
+                        for (uint32_t c = 0; c < 8; ++c) {
+                            const float4 t4 = reinterpret_cast<const float4*>(blk + 32 * hb)[c];
+                            x[4 * c] = t4.x;
+                            x[4 * c + 1] = t4.y;
+                            x[4 * c + 2] = t4.z;
+                            x[4 * c + 3] = t4.w;
+                        }
+                    } else {
+#pragma unroll
+                        for (uint32_t l = 0; l < 32; ++l) x[l] = blk[32 * hb + l];
+                    }
+#pragma unroll
+                    for (uint32_t r = 0; r < 2; ++r)
+#pragma unroll
+                        for (uint32_t l = 0; l < 16; ++l) {
+                            const float df = qh[32 * hb + 16 * r + l] - x[16 * r + l];
+                            p[l] = __builtin_fmaf(df, df, (hb == 0 && r == 0) ? 0.0f : p[l]);
+                        }
+                }
+                return tree16_regs(p);
+            }
 #pragma unroll 1
             for (uint32_t r = 0; r < 4; ++r) {  // dims 16r .. 16r+15: one step of each lane's chain
                 float x[16];
@@ -660,11 +720,23 @@ struct Scanner {
         }
         __syncthreads();
         if (h) {
-            const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16;
-            for (uint32_t i = g; i < n; i += G) {
+            // one thread per vector; the block distances of a vector are independent, so their
+            // loads are issued together (two U8 blocks per round trip) and added in block order
+            const uint32_t nhs = h / kHDim;
+            for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
                 dist_t dd = dist[i];
-                for (uint32_t hd = 0; hd < h; hd += kHDim) dd += horizontal16(i, hd, l);
-                if (l == 0) dist[i] = dd;
+                if constexpr (U8) {
+                    for (uint32_t b = 0; b < nhs; b += 2) {
+                        uint32_t x0[16], x1[16];
+                        fetch_block_u8(i, b * kHDim, x0);
+                        if (b + 1 < nhs) fetch_block_u8(i, (b + 1) * kHDim, x1);
+                        dd += block_dist_u8(x0, b * kHDim);
+                        if (b + 1 < nhs) dd += block_dist_u8(x1, (b + 1) * kHDim);
+                    }
+                } else {
+                    for (uint32_t hd = 0; hd < h; hd += kHDim) dd += horizontal_thread(i, hd);
+                }
+                dist[i] = dd;
             }
             __syncthreads();
         }
@@ -747,35 +819,37 @@ struct Scanner {
         }
         const uint32_t need = static_cast<uint32_t>(ceilf(0.8f * static_cast<float>(n)));
         if constexpr (SUMM) {
-            // every checkpoint value is at hand: count all of them in one pass, then take the
-            // first checkpoint where the loop below would stop
+            // every checkpoint value is at hand: count all of them (ballots, four checkpoints
+            // in flight), then take the first checkpoint where the loop below would stop
             const uint32_t nw = a.nw;
             if (nw == 0) return 0;
             for (uint32_t t = threadIdx.x; t < nw; t += BLOCK) wcnt[t] = 0;
             __syncthreads();
-            uint32_t c[kMaxCheckpoints];
-            dist_t tau[kMaxCheckpoints];
             uint32_t e = 0;
+            for (uint32_t t0 = 0; t0 < nw; t0 += 4) {
+                dist_t tau[4];
+                uint32_t cnt[4];
 #pragma unroll
-            for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
-                c[t] = 0;
-                if (t < nw) {
-                    e = min(e + c_fetch[t], v);
-                    tau[t] = threshold(k, e);
+                for (uint32_t u = 0; u < 4; ++u) {
+                    cnt[u] = 0;
+                    if (t0 + u < nw) {
+                        e = min(e + c_fetch[t0 + u], v);
+                        tau[u] = threshold(k, e);
+                    }
                 }
-            }
-            for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+                for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
+                    const uint32_t i = i0 + threadIdx.x;
+                    float x[4];
 #pragma unroll
-                for (uint32_t t = 0; t < kMaxCheckpoints; ++t)
-                    if (t < nw) c[t] += sb[(size_t)t * n + i] >= tau[t];
-            }
+                    for (uint32_t u = 0; u < 4; ++u) x[u] = (t0 + u < nw && i < n) ? sb[(size_t)(t0 + u) * n + i] : 0.0f;
 #pragma unroll
-            for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
-                if (t < nw) {
-                    uint32_t x = c[t];
+                    for (uint32_t u = 0; u < 4; ++u)
+                        cnt[u] += static_cast<uint32_t>(__popcll(__ballot(t0 + u < nw && i < n && x[u] >= tau[u])));
+                }
+                if ((threadIdx.x & (kWave - 1)) == 0) {
 #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-                    if ((threadIdx.x & (kWave - 1)) == 0 && x) atomicAdd(&wcnt[t], x);
+                    for (uint32_t u = 0; u < 4; ++u)
+                        if (t0 + u < nw && cnt[u]) atomicAdd(&wcnt[t0 + u], cnt[u]);
                 }
             }
             __syncthreads();
@@ -790,7 +864,7 @@ struct Scanner {
                     break;
                 }
             }
-            const float* vt = sb + (size_t)tw * n;
+            const float* vt = sb + (size_t)tw * n;  // just read: served by the caches
             for (uint32_t i = threadIdx.x; i < n; i += BLOCK) dist[i] = vt[i];
             __syncthreads();
             return prev;
@@ -799,7 +873,7 @@ struct Scanner {
         while (npruned < need && cur < v) {
             const uint32_t end = min(cur + c_fetch[t], v);
             const dist_t tau = threshold(k, end);
-            uint32_t cnt = 0;
+            uint32_t cnt = 0;  // wave-uniform
             for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
                 dist_t acc[kVT];
 #pragma unroll
@@ -808,13 +882,11 @@ struct Scanner {
 #pragma unroll
                 for (uint32_t u = 0; u < kVT; ++u) {
                     uint32_t i = i0 + u * BLOCK + threadIdx.x;
-                    if (i < n) {
-                        dist[i] = acc[u];
-                        cnt += acc[u] >= tau;
-                    }
+                    if (i < n) dist[i] = acc[u];
+                    cnt += static_cast<uint32_t>(__popcll(__ballot(i < n && acc[u] >= tau)));
                 }
             }
-            npruned = block_sum<BLOCK>(cnt, red);
+            npruned = block_sum_uniform<BLOCK>(cnt, red);
             bytes += (unsigned long long)n * (end - cur) * (U8 ? 1 : 4);
             cur = end;
             ++t;
