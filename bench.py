@@ -1,14 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark of the PDX search path on one MI355X.
 
-Default workload (--config c2): IVF float32 with ADSampling. N = 1,000,000 unit-norm vectors of
-d = 768 from a 1,024-component Gaussian mixture (centres ~ N(0, I) normalised, noise norm 0.15,
-rows renormalised), 1,000 IVF lists, 10,000 queries from the mixture, k = 10, nprobe = 32, L2.
+Default workload (--config c3): two-level IVF SQ8 (IndexPDXIVFTreeSQ8). N = 5,000,000, d = 1,024,
+4,096-component mixture (centres ~ N(0, I), per-coordinate sigma 0.2), normalised as the Python
+API does by default, 17,889 lists / 133 meso clusters, k = 20, batches of 4,096 queries with
+per-query nprobe drawn uniformly from [8, 128]; results are checked bit-exact against the oracle
+in the run.
 
---config c3: two-level IVF SQ8 (IndexPDXIVFTreeSQ8): N = 5,000,000, d = 1,024, 4,096-component
-mixture (centres ~ N(0, I), per-coordinate sigma 0.2), normalised as the Python API does by
-default, 17,889 lists / 133 meso clusters, k = 20, batches of 4,096 queries with per-query nprobe
-drawn uniformly from [8, 128]; results are checked bit-exact against the oracle in the run.
+--config c2: IVF float32 with ADSampling. N = 1,000,000 unit-norm vectors of d = 768 from a
+1,024-component Gaussian mixture (centres ~ N(0, I) normalised, noise norm 0.15, rows
+renormalised), 1,000 IVF lists, 10,000 queries from the mixture, k = 10, nprobe = 32, L2.
 
 A step is one batched search of --batch queries through the device-buffer C-ABI
 (pdxg_search_device) with queries, index and outputs resident in HBM. The JSON line also carries
@@ -31,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 (vector, with packed math)
 
 CONFIGS = {
     "c2": dict(n=1_000_000, d=768, centers=1024, sigma=0.15, unit=True, kind="pdx_f32", nlist=1000, n_meso=0,
@@ -102,7 +104,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     for key in ("n", "d", "centers", "nlist", "nq", "batch", "k", "nprobe"):
         ap.add_argument("--" + key, type=int, default=None)
     ap.add_argument("--sigma", type=float, default=None)
@@ -111,7 +113,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work per baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", default="1,2,4",
+                    help="batches in flight (server slots); a list runs the serving leg once per value")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--sigma-per-coord", action="store_true", help="unit-norm mixtures: sigma per coordinate")
     ap.add_argument("--target-recall", type=float, default=0.95)
@@ -209,6 +212,8 @@ def main():
             torch.cuda.synchronize()
             ids[s:s + m] = out_ids[:m].cpu().numpy().view(np.uint32)
             cnt[s:s + m] = out_c[:m].cpu().numpy()
+        k10 = min(10, K)
+        state["recall10"] = recall(ids[:, :k10], np.minimum(cnt, k10), gt, k10)
         return recall(ids, cnt, gt, K)
 
     sweep = {}
@@ -216,7 +221,7 @@ def main():
         set_nprobe(nprobe_q)
         key = str(int(nprobe_q[0])) if not cfg["nprobe_range"] else "U[%d,%d]" % cfg["nprobe_range"]
         sweep[key] = recall_now()
-        log(f"recall@{K} = {sweep[key]:.4f} at nprobe {key}")
+        log(f"recall@{K} = {sweep[key]:.4f}, recall@10 = {state['recall10']:.4f} at nprobe {key}")
         if cfg["nprobe_range"] or sweep[key] >= args.target_recall or nprobe_q[0] >= nlist:
             break
         nprobe_q[:] = min(nlist, int(nprobe_q[0]) * 2)
@@ -229,7 +234,8 @@ def main():
                           out_d.data_ptr(), out_i.data_ptr(), out_c.data_ptr(), stream=sp, k_all=K)
 
     plan = tune(state["ws"], B)
-    log(f"autotuned scan block sizes (meso, data): {plan}")
+    log(f"autotuned scan block sizes (meso, data): {plan}"
+        + (f"; two-phase meso scan: {state['ws'].meso_summary}" if tree else ""))
 
     parity = None
     if args.parity_queries:
@@ -243,35 +249,67 @@ def main():
     torch.cuda.synchronize()
     stage = np.zeros(4)
     sbytes = np.zeros(2)
+    meso = np.zeros(2)
+    pairs = 0
     t0 = time.perf_counter()
     ws = state["ws"]
     for i in range(args.steps):
         run((i % nb) * B, B)
         stage += ws.stage_times()
         sbytes += ws.stage_bytes()
+        if tree:
+            meso += ws.meso_times()
+            pairs += ws.meso_pairs()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     qps = B * args.steps / elapsed
     stage /= args.steps
     sbytes /= args.steps
+    meso /= args.steps
+    pairs /= args.steps
     log(f"batched: {ms_per_step:.3f} ms/step, {qps:.0f} QPS; stage ms preprocess/probe/meso/scan = {stage.round(4)}; "
-        f"scan bytes meso/data = {sbytes[0]:.3e}/{sbytes[1]:.3e}")
-    # dominant kernel: the longer of the two scans (the meso scan only exists for two-level indexes)
-    dom = 1 if stage[3] >= stage[2] else 0
-    dom_ms = float(stage[3 if dom else 2])
-    achieved = sbytes[dom] / (dom_ms * 1e-3) / 1e9
+        f"scan bytes meso/data = {sbytes[0]:.3e}/{sbytes[1]:.3e}"
+        + (f"; meso phase1/replay ms = {meso.round(4)}, pairs {pairs:.4g}" if tree else ""))
+    # Kernels timed with HIP events on the search stream: the data-level scan (HBM-bound:
+    # algorithmic bytes = what the reference's schedule reads), and for two-level indexes the
+    # meso stage, split into the two-phase scan's summary phase (FP32 vector-ALU bound:
+    # 3 FLOP per (query, member, dim) plus a 15-add tree per 64-dim block) and the replay.
     u8 = cfg["kind"].endswith("u8")
-    kernel = (("k_scan<U8>" if u8 else "k_scan<F32>") + " (data-level pruned PDX scan)") if dom else \
-        "k_scan<F32> (meso-level pruned PDX scan)"
+    scan_ms = float(stage[3])
+    scan_gbs = sbytes[1] / (scan_ms * 1e-3) / 1e9
+    scan_roof = {"bound": "hbm", "kernel": ("k_scan<U8>" if u8 else "k_scan<F32>") + " (data-level pruned PDX scan)",
+                 "achieved": round(scan_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(scan_gbs / HBM_PEAK_GBS, 4), "ms": round(scan_ms, 4),
+                 "algorithmic_bytes_per_launch": int(sbytes[1])}
+    roofs = {"data_scan": scan_roof}
+    if tree and pairs > 0:
+        flops = pairs * (3 * cfg["d"] + 15 * (idx.l1.h // 64))
+        tf = flops / (meso[0] * 1e-3) / 1e12
+        roofs["meso_phase1"] = {"bound": "valu", "kernel": "k_meso_summary (two-phase meso scan, phase 1)",
+                                "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": round(tf / FP32_PEAK_TFLOPS, 4), "ms": round(float(meso[0]), 4),
+                                "pairs_per_launch": int(pairs), "flops_per_launch": int(flops)}
+        roofs["meso_replay"] = {"kernel": "k_scan<F32, replay> (two-phase meso scan, phase 2)",
+                                "ms": round(float(meso[1]), 4),
+                                "reference_algorithmic_bytes": int(sbytes[0])}
+    elif tree:
+        mgbs = sbytes[0] / (float(stage[2]) * 1e-3) / 1e9
+        roofs["meso_scan"] = {"bound": "hbm", "kernel": "k_scan<F32> (meso-level pruned PDX scan)",
+                              "achieved": round(mgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(mgbs / HBM_PEAK_GBS, 4), "ms": round(float(stage[2]), 4),
+                              "algorithmic_bytes_per_launch": int(sbytes[0])}
+    dom_key = max((k for k in roofs if "bound" in roofs[k]), key=lambda k: roofs[k]["ms"])
     workload = (f"{cfg['kind']}, N={cfg['n']}, d={cfg['d']}, {nlist} lists, {args.metric}, k={K}, nprobe={key}, "
                 f"batch={B}")
-    traffic = None  # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile of this workload
+    # HBM bytes per launch of each kernel from the committed FETCH_SIZE / WRITE_SIZE profiles
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            traffic = json.load(f).get(workload, {}).get("traffic_bytes_per_launch")
+            tr = json.load(f).get(workload, {})
     except (OSError, ValueError):
-        pass
+        tr = {}
+    for kname, r in roofs.items():
+        r["traffic"] = tr.get(kname)
 
     # batch-64 latency and the plain per-query path (one query per call)
     ws64 = workspace(64, state["np_max"])
@@ -300,7 +338,13 @@ def main():
     log(f"plain per-query: {plain_qps:.0f} QPS p50 {np.percentile(lat1, 50):.3f} ms p99 {np.percentile(lat1, 99):.3f} ms; "
         f"batch-64 p50 {np.percentile(lat64, 50):.3f} ms")
 
-    serving = None if args.no_serve else serve_leg(_capi, g, q, nprobe_q, cfg, args, plan)
+    serving = None
+    if not args.no_serve:
+        # one serving leg per slot count; the headline is the one with the lowest p99 at 50% load
+        legs = [serve_leg(_capi, g, q, nprobe_q, cfg, args, plan, int(ns)) for ns in str(args.serve_slots).split(",")]
+        serving = min(legs, key=lambda l: l["load"]["50%"]["p99_ms"])
+        serving["by_slots"] = {str(l["slots"]): {"saturated_qps": l["saturated_qps"],
+                                                 "p99_ms_at_50%": l["load"]["50%"]["p99_ms"]} for l in legs}
     cpu = None if args.no_cpu else cpu_baseline(idx, q, nprobe_q, K, args)
 
     line = {
@@ -317,8 +361,11 @@ def main():
         "dtype": "u8 (int32 accumulation)" if u8 else "f32",
         "data": "synthetic (seeded Gaussian mixture; index built in-process)",
         "config": {"workload": workload, "global_batch": B, "parallelism": "single GPU",
-                   "scan_block": {"batch": list(plan), "batch64": list(plan64), "one_query": list(plan1)}},
+                   "scan_block": {"batch": list(plan), "batch64": list(plan64), "one_query": list(plan1)},
+                   "meso_two_phase": {"batch": ws.meso_summary, "batch64": ws64.meso_summary,
+                                      "one_query": ws1.meso_summary} if tree else None},
         f"recall_at_{K}": round(rec, 4),
+        "recall_at_10": round(state["recall10"], 4),
         "recall_sweep": sweep,
         "parity_vs_oracle": parity,
         "plain_per_query_qps": round(plain_qps, 1),
@@ -330,10 +377,8 @@ def main():
                        f"batch{B}_mean": round(ms_per_step, 4)},
         "stage_ms": {"preprocess": round(float(stage[0]), 4), "probe": round(float(stage[1]), 4),
                      "meso_scan": round(float(stage[2]), 4), "scan": round(float(stage[3]), 4)},
-        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": int(sbytes[dom]),
-                     "meso_scan_bytes": int(sbytes[0]), "scan_bytes": int(sbytes[1])},
+        "roofline": dict(roofs[dom_key], stage=dom_key),
+        "kernels": roofs,
         "serving": serving,
         "cpu_baseline": cpu,
         "build_s": round(build_s, 1),
@@ -365,10 +410,10 @@ def check_parity(idx, q, nprobe_q, K, n, run, out_ids, out_d, out_i, out_c, torc
     return {"queries": n, "ids_equal": ok_i, "int32_distances_equal": ok_d, "float_distances_equal": ok_f}
 
 
-def serve_leg(_capi, g, q, nprobe_q, cfg, args, plan):
+def serve_leg(_capi, g, q, nprobe_q, cfg, args, plan, slots):
     """Continuous batching: saturated QPS, then Poisson arrivals at fractions of it (p50/p99)."""
     K = cfg["k"]
-    sv = _capi.Server(g, max_batch=cfg["batch"], max_k=K, max_nprobe=int(nprobe_q.max()), max_slots=args.serve_slots,
+    sv = _capi.Server(g, max_batch=cfg["batch"], max_k=K, max_nprobe=int(nprobe_q.max()), max_slots=slots,
                       scan_block=plan)
     rng = np.random.default_rng(7)
     n_sat = min(len(q), 8 * cfg["batch"])
@@ -386,7 +431,7 @@ def serve_leg(_capi, g, q, nprobe_q, cfg, args, plan):
         out["load"][f"{int(frac * 100)}%"] = {"offered_qps": round(rate, 1), "served_qps": round(n / el, 1),
                                               "p50_ms": round(float(np.percentile(lat, 50)), 4),
                                               "p99_ms": round(float(np.percentile(lat, 99)), 4)}
-        log(f"serving at {int(frac * 100)}% of {sat:.0f} QPS: p50 {np.percentile(lat, 50):.3f} ms "
+        log(f"serving ({slots} slots) at {int(frac * 100)}% of {sat:.0f} QPS: p50 {np.percentile(lat, 50):.3f} ms "
             f"p99 {np.percentile(lat, 99):.3f} ms")
     st = sv.stats()
     out["mean_batch"] = round(st["completed"] / max(1, st["batches"]), 1)

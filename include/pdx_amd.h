@@ -129,6 +129,14 @@ pdxg_status pdxg_workspace_stage_times(pdxg_workspace* ws, float* out_ms4);
 /* Algorithmic bytes read by [0] the meso-level scan and [1] the data-level scan of the last timed
  * search (summed over its queries; centroid probe bytes excluded). */
 pdxg_status pdxg_workspace_stage_bytes(pdxg_workspace* ws, uint64_t* out2);
+/* Split of the meso-level stage of the last timed search: [0] the two-phase scan's first phase
+ * (visit-list inversion and partial-distance summaries), [1] the scan proper (the replay, or the
+ * pruned scan when the two-phase scan did not run, with [0] = 0). Not in the reference. */
+pdxg_status pdxg_workspace_meso_times(pdxg_workspace* ws, float* out2);
+/* (query, meso member) pairs summarised by the first phase of the last timed search (0 when the
+ * two-phase scan did not run): each costs d subtractions, d fused multiply-adds and, per 64-dim
+ * horizontal block, a 15-add reduction. Not in the reference. */
+pdxg_status pdxg_workspace_meso_pairs(pdxg_workspace* ws, uint64_t* out1);
 
 /* Scan workgroup size per level ("waves per cluster": 64 .. 1024 threads per query; 0 = the
  * built-in heuristic). pdxg_workspace_autotune runs the given search (device buffers, as for

@@ -76,7 +76,7 @@ EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster
            "pdxg_search", "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_workspace_bytes",
            "pdxg_search_device", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
            "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times",
-           "pdxg_workspace_stage_bytes",
+           "pdxg_workspace_stage_bytes", "pdxg_workspace_meso_times", "pdxg_workspace_meso_pairs",
            "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune",
            "pdxg_workspace_set_meso_summary", "pdxg_workspace_meso_summary", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
            "pdxg_server_get_stats", "pdxg_server_replay"]
@@ -112,6 +112,10 @@ def lib():
     L.pdxg_workspace_enable_timing.argtypes = [vp, ctypes.c_int]
     L.pdxg_workspace_stage_times.argtypes = [vp, f32p]
     L.pdxg_workspace_stage_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.pdxg_workspace_meso_times.argtypes = [vp, f32p]
+    L.pdxg_workspace_meso_times.restype = ctypes.c_int
+    L.pdxg_workspace_meso_pairs.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.pdxg_workspace_meso_pairs.restype = ctypes.c_int
     L.pdxg_workspace_set_plan.argtypes = [vp, u32, u32]
     L.pdxg_workspace_get_plan.argtypes = [vp, u32p]
     L.pdxg_workspace_autotune.argtypes = [vp, vp, ctypes.POINTER(SearchArgs), vp]
@@ -319,6 +323,18 @@ class Workspace:
                        out_counts or None, None)
         check(lib().pdxg_workspace_autotune(self.index.handle, self._h, ctypes.byref(a), ctypes.c_void_p(stream or 0)))
         return self.plan()
+
+    def meso_times(self):
+        """Milliseconds of [two-phase first phase, meso scan / replay] of the last timed search."""
+        out = np.zeros(2, np.float32)
+        check(lib().pdxg_workspace_meso_times(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def meso_pairs(self) -> int:
+        """(query, meso member) pairs summarised by the two-phase scan in the last timed search."""
+        out = np.zeros(1, np.uint64)
+        check(lib().pdxg_workspace_meso_pairs(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return int(out[0])
 
     def stage_bytes(self):
         """Algorithmic bytes of [meso scan, data scan] of the last search."""

@@ -261,7 +261,7 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
     p.o_op = take(kmax * 4);
     p.o_cp = take(block * 4);
     p.o_red = take(64);
-    p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + kMaxCheckpoints + 1) * 4);
+    p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + 2 * kMaxCheckpoints + 1) * 4);
     p.o_rat = take(L.d * 4);
     p.o_ring = take(kRing * sizeof(ClusterMeta));
     p.bytes = (off + 15) / 16 * 16;
@@ -275,9 +275,12 @@ struct pdxg_workspace {
     bool timing = false;
     DevBuf<unsigned long long> stage_bytes;  // [meso scan, data scan] of the last search when timing
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_summ = nullptr;  // end of the two-phase meso scan's first phase
+    bool last_summ = false;        // the last timed search ran the two-phase meso scan
     ~pdxg_workspace() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (ev_summ) (void)hipEventDestroy(ev_summ);
     }
     void mark(int i, hipStream_t st) {
         if (timing) (void)hipEventRecord(ev[i], st);
@@ -513,6 +516,7 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     m.region = ws.region;
     m.nw = ws.nw;
     m.ns = ws.ns;
+    m.pairs = ws.timing && ws.stage_bytes.p ? ws.stage_bytes.p + 2 : nullptr;
     require((uint64_t)nq * ws.order_stride < (1ull << 32), "too many queries x meso clusters for one search");
     hipLaunchKernelGGL(k_fill_u32, dim3((L.nc + 255) / 256), dim3(256), 0, st, ws.m_cnt.p, L.nc, 0u);
     const size_t pairs = (size_t)nq * ws.nv0_max;
@@ -591,9 +595,9 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
     ws.mark(0, st);
     unsigned long long* sb = nullptr;
     if (ws.timing) {
-        if (!ws.stage_bytes.p) ws.stage_bytes.alloc(2);
+        if (!ws.stage_bytes.p) ws.stage_bytes.alloc(3);
         sb = ws.stage_bytes.p;
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, sb, 2u, 0ull);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, sb, 3u, 0ull);
         HIP_CHECK(hipGetLastError());
     }
     launch_preprocess(ix, ws, a.queries, nq, a.queries_transformed != 0, st);
@@ -671,8 +675,10 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s0.stage_bytes = sb;
         const bool summ = ws.summ_ok && ws.summ_use && ix->l0.tomb_total == 0 &&
                           ix->l0.top_prefix[ws.nv0_max] <= ws.region;
+        ws.last_summ = summ;
         if (summ) {
             launch_meso_summary(ix, ws, nq, st);
+            if (ws.timing) (void)hipEventRecord(ws.ev_summ, st);
             s0.summ = ws.summ.p;
             s0.soff = ws.m_soff.p;
             s0.region = ws.region;
@@ -852,9 +858,11 @@ pdxg_status pdxg_workspace_enable_timing(pdxg_workspace* ws, int enable) {
     return guarded([&] {
         require(ws != nullptr, "null argument");
         DeviceGuard g(ws->device);
-        if (enable)
+        if (enable) {
             for (auto& e : ws->ev)
                 if (!e) HIP_CHECK(hipEventCreate(&e));
+            if (!ws->ev_summ) HIP_CHECK(hipEventCreate(&ws->ev_summ));
+        }
         ws->timing = enable != 0;
     });
 }
@@ -869,6 +877,22 @@ pdxg_status pdxg_workspace_stage_times(pdxg_workspace* ws, float* out) {
     });
 }
 
+pdxg_status pdxg_workspace_meso_times(pdxg_workspace* ws, float* out) {
+    return guarded([&] {
+        require(ws && out, "null argument");
+        require(ws->timing, "timing is not enabled on this workspace");
+        DeviceGuard g(ws->device);
+        HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+        out[0] = out[1] = 0.0f;
+        if (ws->last_summ) {
+            HIP_CHECK(hipEventElapsedTime(&out[0], ws->ev[2], ws->ev_summ));
+            HIP_CHECK(hipEventElapsedTime(&out[1], ws->ev_summ, ws->ev[3]));
+        } else {
+            HIP_CHECK(hipEventElapsedTime(&out[1], ws->ev[2], ws->ev[3]));
+        }
+    });
+}
+
 pdxg_status pdxg_workspace_stage_bytes(pdxg_workspace* ws, uint64_t* out) {
     return guarded([&] {
         require(ws && out, "null argument");
@@ -876,6 +900,17 @@ pdxg_status pdxg_workspace_stage_bytes(pdxg_workspace* ws, uint64_t* out) {
         DeviceGuard g(ws->device);
         HIP_CHECK(hipEventSynchronize(ws->ev[4]));
         HIP_CHECK(hipMemcpy(out, ws->stage_bytes.p, 16, hipMemcpyDeviceToHost));
+    });
+}
+
+pdxg_status pdxg_workspace_meso_pairs(pdxg_workspace* ws, uint64_t* out) {
+    return guarded([&] {
+        require(ws && out, "null argument");
+        require(ws->timing && ws->stage_bytes.p, "timing is not enabled on this workspace");
+        DeviceGuard g(ws->device);
+        HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+        *out = 0;
+        if (ws->last_summ) HIP_CHECK(hipMemcpy(out, ws->stage_bytes.p + 2, 8, hipMemcpyDeviceToHost));
     });
 }
 
@@ -915,6 +950,7 @@ pdxg_status pdxg_workspace_autotune(pdxg_index* ix, pdxg_workspace* ws, const pd
         if (!was_timing) {
             for (auto& e : ws->ev)
                 if (!e) HIP_CHECK(hipEventCreate(&e));
+            if (!ws->ev_summ) HIP_CHECK(hipEventCreate(&ws->ev_summ));
             ws->timing = true;
         }
         // tune the meso level first (two-level indexes), then the data level; each candidate runs
@@ -924,8 +960,9 @@ pdxg_status pdxg_workspace_autotune(pdxg_index* ix, pdxg_workspace* ws, const pd
             float best = 1e30f;
             uint32_t best_b = 0;
             bool best_summ = false;
+            const bool summ_before = ws->summ_use;
             for (int summ = (lvl == 0 && ws->summ_ok) ? 1 : 0; summ >= 0; --summ) {
-                ws->summ_use = summ != 0;
+                if (lvl == 0) ws->summ_use = summ != 0;
                 for (uint32_t b : kBlocks) {
                     ws->block[lvl] = b;
                     float ms[4] = {0, 0, 0, 0};
@@ -951,7 +988,7 @@ pdxg_status pdxg_workspace_autotune(pdxg_index* ix, pdxg_workspace* ws, const pd
             }
             require(best_b != 0, "no scan block size fits this index", PDXG_ERR_UNSUPPORTED);
             ws->block[lvl] = best_b;
-            if (lvl == 0) ws->summ_use = best_summ;
+            ws->summ_use = lvl == 0 ? best_summ : summ_before;
         }
         ws->timing = was_timing;
     });

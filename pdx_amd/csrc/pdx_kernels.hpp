@@ -313,6 +313,7 @@ struct Scanner {
     float* rat;       // ADSampling ratio table [d]
     ClusterMeta* ring;
     uint32_t* wcnt;   // SUMM: pruned counts per warmup checkpoint
+    uint32_t* cpe;    // warmup checkpoint ends (cumulative fetch sizes clipped to v)
     uint32_t d, v, h, nvw;
     uint32_t hl = 0;
     unsigned long long bytes = 0;        // uniform: identical in every thread
@@ -350,6 +351,7 @@ struct Scanner {
         h = a.L.h;
         nvw = (v + 3) / 4;
         wcnt = reinterpret_cast<uint32_t*>(taus) + (h / kHDim + (v + kHDim - 1) / kHDim + 1);
+        cpe = wcnt + kMaxCheckpoints + 1;
     }
 
     static __device__ __forceinline__ dist_t mask_value() {
@@ -383,6 +385,13 @@ struct Scanner {
 
     __device__ void load_query(uint32_t q) {
         for (uint32_t j = threadIdx.x; j < d; j += BLOCK) rat[j] = a.ratios[j];
+        if (threadIdx.x == 0) {
+            uint32_t e = 0;
+            for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
+                e = min(e + c_fetch[t], v);
+                cpe[t] = e;
+            }
+        }
         if constexpr (!U8) {
             for (uint32_t j = threadIdx.x; j < d; j += BLOCK) qf[j] = a.qf[(size_t)q * d + j];
         } else {
@@ -825,45 +834,36 @@ struct Scanner {
             if (nw == 0) return 0;
             for (uint32_t t = threadIdx.x; t < nw; t += BLOCK) wcnt[t] = 0;
             __syncthreads();
-            uint32_t e = 0;
-            for (uint32_t t0 = 0; t0 < nw; t0 += 4) {
-                dist_t tau[4];
-                uint32_t cnt[4];
+            for (uint32_t t0 = 0; t0 < nw; t0 += 8) {
+                dist_t tau[8];
+                uint32_t cnt[8];
 #pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
+                for (uint32_t u = 0; u < 8; ++u) {
                     cnt[u] = 0;
-                    if (t0 + u < nw) {
-                        e = min(e + c_fetch[t0 + u], v);
-                        tau[u] = threshold(k, e);
-                    }
+                    if (t0 + u < nw) tau[u] = threshold(k, cpe[t0 + u]);
                 }
                 for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
                     const uint32_t i = i0 + threadIdx.x;
-                    float x[4];
+                    float x[8];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) x[u] = (t0 + u < nw && i < n) ? sb[(size_t)(t0 + u) * n + i] : 0.0f;
+                    for (uint32_t u = 0; u < 8; ++u) x[u] = (t0 + u < nw && i < n) ? sb[(size_t)(t0 + u) * n + i] : 0.0f;
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u)
+                    for (uint32_t u = 0; u < 8; ++u)
                         cnt[u] += static_cast<uint32_t>(__popcll(__ballot(t0 + u < nw && i < n && x[u] >= tau[u])));
                 }
                 if ((threadIdx.x & (kWave - 1)) == 0) {
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u)
+                    for (uint32_t u = 0; u < 8; ++u)
                         if (t0 + u < nw && cnt[u]) atomicAdd(&wcnt[t0 + u], cnt[u]);
                 }
             }
             __syncthreads();
-            uint32_t tw = nw - 1, prev = 0;
-            e = 0;
-            for (uint32_t t = 0; t < nw; ++t) {
-                e = min(e + c_fetch[t], v);
-                bytes += (unsigned long long)n * (e - prev) * 4;
-                prev = e;
-                if (wcnt[t] >= need) {
-                    tw = t;
-                    break;
-                }
-            }
+            // first checkpoint whose count reaches `need` (the loop below stops there), else the last
+            const uint32_t lane = threadIdx.x & (kWave - 1);
+            const uint64_t hit = __ballot(lane < nw && wcnt[lane] >= need);
+            const uint32_t tw = hit ? static_cast<uint32_t>(__ffsll(static_cast<long long>(hit)) - 1) : nw - 1;
+            const uint32_t prev = cpe[tw];
+            bytes += (unsigned long long)n * prev * 4;
             const float* vt = sb + (size_t)tw * n;  // just read: served by the caches
             for (uint32_t i = threadIdx.x; i < n; i += BLOCK) dist[i] = vt[i];
             __syncthreads();

@@ -41,6 +41,7 @@ struct MesoArgs {
     unsigned long long region;
     uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector
     uint32_t wg_per_xcd;        // summary grid = 8 x wg_per_xcd workgroups
+    unsigned long long* pairs;  // optional: summarised (query, vector) pairs of the launch
 };
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
@@ -103,6 +104,7 @@ __global__ __launch_bounds__(256) void k_meso_soff(MesoArgs a) {
         if (r < nv) a.soff[(size_t)q * a.stride + r] = base + ex;
         base += tot;
     }
+    if (a.pairs && threadIdx.x == 0) atomicAdd(a.pairs, (unsigned long long)base);
 }
 
 // Query rows are read with wave-uniform addresses through the scalar cache.
@@ -114,78 +116,70 @@ __device__ __forceinline__ const_float* to_const(const float* p) {
 #pragma clang diagnostic pop
 }
 
-// Phase 1. Summary block of (query, rank), component-major: value t of vector i at
-// [t * n + i]; t < nw: vertical chain over dims [0, end_t); t = nw + b: horizontal block b.
-__global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
-    const uint32_t wg = (blockIdx.x % 8) * a.wg_per_xcd + blockIdx.x / 8;  // items of an XCD are contiguous
-    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+// Phase 1 body for NQ (compile-time) queries of one work item on one wave: the cluster's vectors
+// one per lane in chunks of 64, every query's chain per lane.
+template <uint32_t NQ>
+__device__ __forceinline__ void meso_summarize(const MesoArgs& a, const float* data, uint32_t S, uint32_t n,
+                                               const_float* qbase, const uint32_t (&qo)[kMesoQ],
+                                               const uint32_t (&ob)[kMesoQ]) {
     const uint32_t lane = threadIdx.x % kWave;
-    const uint32_t item = wg * kMesoWaves + w;
-    const uint32_t nc = a.L.nc;
-    if (item >= a.cstart[nc]) return;  // wave-uniform; the kernel has no barriers
-    uint32_t lo = 0, hi = nc;          // cstart[lo] <= item < cstart[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.cstart[mid] <= item) lo = mid;
-        else hi = mid;
-    }
-    const uint32_t m = lo;
-    const uint32_t e0 = a.lstart[m] + (item - a.cstart[m]) * kMesoQ;
-    const uint32_t ne = min(kMesoQ, a.lstart[m + 1] - e0);
-    const uint32_t n = a.L.n_used[m];
-    if (n == 0) return;
-    const uint32_t d = a.L.d, v = a.L.v, nh = a.L.h / kHDim, S = a.L.cap[m];
-    const float* data = reinterpret_cast<const float*>(a.L.data + a.L.data_off[m]);
+    const uint32_t v = a.L.v, nh = a.L.h / kHDim;
     const bool aligned = ((size_t)v * S) % 4 == 0;  // 16-byte aligned horizontal blocks
-
-    // per query: row offset of the rotated query and first summarised vector of its block
-    // (region index q * region + soff < 2^32, checked by the host); ~0u = not stored
-    const_float* qbase = to_const(a.qf);
-    uint32_t qo[kMesoQ], ob[kMesoQ];
+    // one 64-bit base per query and constant offsets within a step, so the query values come in
+    // as multi-dword scalar loads with no per-value address arithmetic
+    const_float* qp[NQ];
 #pragma unroll
-    for (uint32_t u = 0; u < kMesoQ; ++u) {
-        const bool ok = u < ne;
-        const uint32_t e = __builtin_amdgcn_readfirstlane(a.list[e0 + (ok ? u : 0u)]);
-        const uint32_t q = e / a.stride;
-        const uint32_t so = __builtin_amdgcn_readfirstlane(a.soff[e]);
-        qo[u] = q * d;
-        ob[u] = (ok && (unsigned long long)so + n <= a.region) ? static_cast<uint32_t>(q * a.region + so) : ~0u;
-    }
-
+    for (uint32_t u = 0; u < NQ; ++u) qp[u] = qbase + qo[u];
     for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
         const uint32_t i = i0 + lane;
         const bool valid = i < n;
         const uint32_t ii = valid ? i : n - 1;
         const float* col = data + ii;
-        float acc[kMesoQ];
+        float acc[NQ];
 #pragma unroll
-        for (uint32_t u = 0; u < kMesoQ; ++u) acc[u] = 0.0f;
+        for (uint32_t u = 0; u < NQ; ++u) acc[u] = 0.0f;
         uint32_t j = 0, end = 0;
         for (uint32_t t = 0; t < a.nw; ++t) {
             end = min(end + c_fetch[t], v);
+            for (; j + 8 <= end; j += 8) {
+                float x[8];
+#pragma unroll
+                for (uint32_t s = 0; s < 8; ++s) x[s] = col[(size_t)(j + s) * S];
+#pragma unroll
+                for (uint32_t u = 0; u < NQ; ++u) {
+                    const_float* pj = qp[u] + j;
+#pragma unroll
+                    for (uint32_t s = 0; s < 8; ++s) {
+                        const float df = pj[s] - x[s];
+                        acc[u] = __builtin_fmaf(df, df, acc[u]);
+                    }
+                }
+            }
             for (; j + 4 <= end; j += 4) {
                 float x[4];
 #pragma unroll
                 for (uint32_t s = 0; s < 4; ++s) x[s] = col[(size_t)(j + s) * S];
 #pragma unroll
-                for (uint32_t u = 0; u < kMesoQ; ++u)
+                for (uint32_t u = 0; u < NQ; ++u) {
+                    const_float* pj = qp[u] + j;
 #pragma unroll
                     for (uint32_t s = 0; s < 4; ++s) {
-                        const float df = qbase[qo[u] + j + s] - x[s];
+                        const float df = pj[s] - x[s];
                         acc[u] = __builtin_fmaf(df, df, acc[u]);
                     }
+                }
             }
             for (; j < end; ++j) {
                 const float x = col[(size_t)j * S];
 #pragma unroll
-                for (uint32_t u = 0; u < kMesoQ; ++u) {
-                    const float df = qbase[qo[u] + j] - x;
+                for (uint32_t u = 0; u < NQ; ++u) {
+                    const float df = qp[u][j] - x;
                     acc[u] = __builtin_fmaf(df, df, acc[u]);
                 }
             }
             if (valid) {
 #pragma unroll
-                for (uint32_t u = 0; u < kMesoQ; ++u)
+                for (uint32_t u = 0; u < NQ; ++u)
                     if (ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)t * n + i] = acc[u];
             }
         }
@@ -205,25 +199,70 @@ __global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
 #pragma unroll
                 for (uint32_t l = 0; l < kHDim; ++l) xb[l] = blk[l];
             }
-            for (uint32_t u = 0; u < kMesoQ; ++u) {
-                const_float* qh = qbase + qo[u] + v + b * kHDim;
+            for (uint32_t u = 0; u < NQ; ++u) {
                 float p[16];
 #pragma unroll
-                for (uint32_t l = 0; l < 16; ++l) {
-                    const float df = qh[l] - xb[l];
-                    p[l] = __builtin_fmaf(df, df, 0.0f);
+                for (uint32_t hf = 0; hf < 2; ++hf) {  // 32 query values (scalar registers) at a time
+                    const_float* qh = qp[u] + v + b * kHDim + 32 * hf;
+#pragma unroll
+                    for (uint32_t r = 0; r < 2; ++r)
+#pragma unroll
+                        for (uint32_t l = 0; l < 16; ++l) {
+                            const float df = qh[16 * r + l] - xb[32 * hf + 16 * r + l];
+                            p[l] = __builtin_fmaf(df, df, (hf == 0 && r == 0) ? 0.0f : p[l]);
+                        }
                 }
-#pragma unroll
-                for (uint32_t r = 1; r < 4; ++r)
-#pragma unroll
-                    for (uint32_t l = 0; l < 16; ++l) {
-                        const float df = qh[16 * r + l] - xb[16 * r + l];
-                        p[l] = __builtin_fmaf(df, df, p[l]);
-                    }
                 const float hv = tree16_regs(p);
                 if (valid && ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)(a.nw + b) * n + i] = hv;
             }
         }
+    }
+}
+
+// Phase 1. Summary block of (query, rank), component-major: value t of vector i at
+// [t * n + i]; t < nw: vertical chain over dims [0, end_t); t = nw + b: horizontal block b.
+__global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
+    const uint32_t wg = (blockIdx.x % 8) * a.wg_per_xcd + blockIdx.x / 8;  // items of an XCD are contiguous
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t item = wg * kMesoWaves + w;
+    const uint32_t nc = a.L.nc;
+    if (item >= a.cstart[nc]) return;  // wave-uniform; the kernel has no barriers
+    uint32_t lo = 0, hi = nc;          // cstart[lo] <= item < cstart[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.cstart[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t m = lo;
+    const uint32_t e0 = a.lstart[m] + (item - a.cstart[m]) * kMesoQ;
+    const uint32_t ne = min(kMesoQ, a.lstart[m + 1] - e0);
+    const uint32_t n = a.L.n_used[m];
+    if (n == 0) return;
+    const uint32_t d = a.L.d, S = a.L.cap[m];
+    const float* data = reinterpret_cast<const float*>(a.L.data + a.L.data_off[m]);
+
+    // per query: row offset of the rotated query and first summarised vector of its block
+    // (region index q * region + soff < 2^32, checked by the host); ~0u = not stored
+    const_float* qbase = to_const(a.qf);
+    uint32_t qo[kMesoQ], ob[kMesoQ];
+#pragma unroll
+    for (uint32_t u = 0; u < kMesoQ; ++u) {
+        const bool ok = u < ne;
+        const uint32_t e = __builtin_amdgcn_readfirstlane(a.list[e0 + (ok ? u : 0u)]);
+        const uint32_t q = e / a.stride;
+        const uint32_t so = __builtin_amdgcn_readfirstlane(a.soff[e]);
+        qo[u] = q * d;
+        ob[u] = (ok && (unsigned long long)so + n <= a.region) ? static_cast<uint32_t>(q * a.region + so) : ~0u;
+    }
+    switch (ne) {  // the last item of a cluster may hold fewer queries
+        case 8: meso_summarize<8>(a, data, S, n, qbase, qo, ob); break;
+        case 7: meso_summarize<7>(a, data, S, n, qbase, qo, ob); break;
+        case 6: meso_summarize<6>(a, data, S, n, qbase, qo, ob); break;
+        case 5: meso_summarize<5>(a, data, S, n, qbase, qo, ob); break;
+        case 4: meso_summarize<4>(a, data, S, n, qbase, qo, ob); break;
+        case 3: meso_summarize<3>(a, data, S, n, qbase, qo, ob); break;
+        case 2: meso_summarize<2>(a, data, S, n, qbase, qo, ob); break;
+        default: meso_summarize<1>(a, data, S, n, qbase, qo, ob); break;
     }
 }
 
