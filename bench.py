@@ -119,6 +119,9 @@ def main():
     ap.add_argument("--sigma-per-coord", action="store_true", help="unit-norm mixtures: sigma per coordinate")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--parity-queries", type=int, default=256)
+    ap.add_argument("--index-cache", default="",
+                    help="index file: loaded when present, else written after the build (e.g. to profile without "
+                         "the build's kernels)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -145,8 +148,15 @@ def main():
                    args.sigma_per_coord)
     log(f"data {x.shape} in {time.time() - t0:.1f}s")
     t0 = time.time()
-    idx = builder.build_index(x, cfg["kind"], num_clusters=cfg["nlist"], num_meso_clusters=cfg["n_meso"],
-                              normalize=cfg["normalize"], distance_metric=args.metric, seed=42, device=0)
+    from pdx_amd import pdxfile
+    if args.index_cache and os.path.exists(args.index_cache):
+        idx = pdxfile.load(args.index_cache)
+        log(f"index loaded from {args.index_cache}")
+    else:
+        idx = builder.build_index(x, cfg["kind"], num_clusters=cfg["nlist"], num_meso_clusters=cfg["n_meso"],
+                                  normalize=cfg["normalize"], distance_metric=args.metric, seed=42, device=0)
+        if args.index_cache:
+            pdxfile.save(idx, args.index_cache)
     build_s = time.time() - t0
     sizes = np.array([c.n_live for c in idx.l1.clusters])
     nlist = idx.l1.nc
@@ -309,7 +319,7 @@ def main():
     except (OSError, ValueError):
         tr = {}
     for kname, r in roofs.items():
-        r["traffic"] = tr.get(kname)
+        r["traffic"] = (tr.get(kname) or {}).get("traffic_bytes_per_launch")
 
     # batch-64 latency and the plain per-query path (one query per call)
     ws64 = workspace(64, state["np_max"])
