@@ -39,6 +39,10 @@ CONFIGS = {
                normalize=False, nq=10_000, batch=1024, k=10, nprobe=32, nprobe_range=None, gt=1000),
     "c3": dict(n=5_000_000, d=1024, centers=4096, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
                normalize=True, nq=8192, batch=4096, k=20, nprobe=0, nprobe_range=(8, 128), gt=512),
+    # serving over a resident two-level SQ8 index; each query draws k and nprobe from the sets
+    "c4": dict(n=10_000_000, d=1536, centers=8192, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
+               normalize=True, nq=8192, batch=4096, k=100, k_choices=(10, 20, 100), nprobe=0, nprobe_range=None,
+               nprobe_choices=(16, 64, 256), gt=256, loads=(0.1, 0.25, 0.5, 0.75, 0.9, 1.0), scratch_gib=24),
 }
 
 
@@ -181,11 +185,17 @@ def main():
 
     B, K, NQ = cfg["batch"], cfg["k"], cfg["nq"]
     rng = np.random.default_rng(args.seed + 1)
-    if cfg["nprobe_range"]:
+    if cfg.get("nprobe_choices"):
+        nprobe_q = rng.choice(np.array(cfg["nprobe_choices"], np.uint32), NQ)
+    elif cfg["nprobe_range"]:
         lo, hi = cfg["nprobe_range"]
         nprobe_q = rng.integers(lo, hi + 1, NQ).astype(np.uint32)
     else:
         nprobe_q = np.full(NQ, cfg["nprobe"], np.uint32)
+    # per-query k (k is the largest when the configuration draws k per query)
+    k_q = rng.choice(np.array(cfg["k_choices"], np.uint32), NQ) if cfg.get("k_choices") else np.full(NQ, K, np.uint32)
+    per_k = bool(cfg.get("k_choices"))
+    kd = torch.from_numpy(k_q.astype(np.int32)).to(dev)
     qd = torch.from_numpy(q).to(dev)
     out_ids = torch.zeros((B, K), dtype=torch.int32, device=dev)
     out_d = torch.zeros((B, K), dtype=torch.float32, device=dev)
@@ -202,8 +212,9 @@ def main():
     def run(s, m, w=None):
         w = w or state["ws"]
         npd = state["np_dev"]
-        w.search(qd[s:s + m].data_ptr(), m, 0, npd[s:s + m].data_ptr(), K, state["np_max"], K,
-                 out_ids.data_ptr(), out_d.data_ptr(), out_i.data_ptr(), out_c.data_ptr(), 0, stream=sp, k_all=K)
+        w.search(qd[s:s + m].data_ptr(), m, kd[s:s + m].data_ptr() if per_k else 0, npd[s:s + m].data_ptr(), K,
+                 state["np_max"], K, out_ids.data_ptr(), out_d.data_ptr(), out_i.data_ptr(), out_c.data_ptr(), 0,
+                 stream=sp, k_all=K)
 
     def set_nprobe(arr):
         state["np_dev"] = torch.from_numpy(arr.astype(np.int32)).to(dev)
@@ -224,15 +235,23 @@ def main():
             cnt[s:s + m] = out_c[:m].cpu().numpy()
         k10 = min(10, K)
         state["recall10"] = recall(ids[:, :k10], np.minimum(cnt, k10), gt, k10)
+        if per_k:  # recall@k with each query's own k
+            return float(np.mean([len(set(ids[i, :cnt[i]].tolist()) & set(gt[i, :k_q[i]].tolist())) / k_q[i]
+                                  for i in range(gtq)]))
         return recall(ids, cnt, gt, K)
 
     sweep = {}
     while True:
         set_nprobe(nprobe_q)
-        key = str(int(nprobe_q[0])) if not cfg["nprobe_range"] else "U[%d,%d]" % cfg["nprobe_range"]
+        if cfg.get("nprobe_choices"):
+            key = "{" + ",".join(str(x) for x in cfg["nprobe_choices"]) + "}"
+        elif cfg["nprobe_range"]:
+            key = "U[%d,%d]" % cfg["nprobe_range"]
+        else:
+            key = str(int(nprobe_q[0]))
         sweep[key] = recall_now()
-        log(f"recall@{K} = {sweep[key]:.4f}, recall@10 = {state['recall10']:.4f} at nprobe {key}")
-        if cfg["nprobe_range"] or sweep[key] >= args.target_recall or nprobe_q[0] >= nlist:
+        log(f"recall@{'k' if per_k else K} = {sweep[key]:.4f}, recall@10 = {state['recall10']:.4f} at nprobe {key}")
+        if cfg["nprobe_range"] or cfg.get("nprobe_choices") or sweep[key] >= args.target_recall or nprobe_q[0] >= nlist:
             break
         nprobe_q[:] = min(nlist, int(nprobe_q[0]) * 2)
     rec = sweep[key]
@@ -240,8 +259,9 @@ def main():
     # autotune the scan workgroup size per level on one batch ("waves per cluster")
     def tune(w, m):
         npd = state["np_dev"]
-        return w.autotune(qd[:m].data_ptr(), m, 0, npd[:m].data_ptr(), K, state["np_max"], K, out_ids.data_ptr(),
-                          out_d.data_ptr(), out_i.data_ptr(), out_c.data_ptr(), stream=sp, k_all=K)
+        return w.autotune(qd[:m].data_ptr(), m, kd[:m].data_ptr() if per_k else 0, npd[:m].data_ptr(), K,
+                          state["np_max"], K, out_ids.data_ptr(), out_d.data_ptr(), out_i.data_ptr(),
+                          out_c.data_ptr(), stream=sp, k_all=K)
 
     plan = tune(state["ws"], B)
     log(f"autotuned scan block sizes (meso, data): {plan}"
@@ -249,7 +269,7 @@ def main():
 
     parity = None
     if args.parity_queries:
-        parity = check_parity(idx, q, nprobe_q, K, min(args.parity_queries, B), run, out_ids, out_d, out_i,
+        parity = check_parity(idx, q, nprobe_q, k_q, min(args.parity_queries, B), run, out_ids, out_d, out_i,
                               out_c, torch)
 
     # timed leg: `steps` batches, inputs resident in HBM, one batch in flight at a time
@@ -310,7 +330,8 @@ def main():
                               "frac": round(mgbs / HBM_PEAK_GBS, 4), "ms": round(float(stage[2]), 4),
                               "algorithmic_bytes_per_launch": int(sbytes[0])}
     dom_key = max((k for k in roofs if "bound" in roofs[k]), key=lambda k: roofs[k]["ms"])
-    workload = (f"{cfg['kind']}, N={cfg['n']}, d={cfg['d']}, {nlist} lists, {args.metric}, k={K}, nprobe={key}, "
+    kdesc = ("{" + ",".join(str(x) for x in cfg["k_choices"]) + "}") if per_k else str(K)
+    workload = (f"{cfg['kind']}, N={cfg['n']}, d={cfg['d']}, {nlist} lists, {args.metric}, k={kdesc}, nprobe={key}, "
                 f"batch={B}")
     # HBM bytes per launch of each kernel from the committed FETCH_SIZE / WRITE_SIZE profiles
     try:
@@ -351,11 +372,11 @@ def main():
     serving = None
     if not args.no_serve:
         # one serving leg per slot count; the headline is the one with the lowest p99 at 50% load
-        legs = [serve_leg(_capi, g, q, nprobe_q, cfg, args, plan, int(ns)) for ns in str(args.serve_slots).split(",")]
+        legs = [serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, int(ns)) for ns in str(args.serve_slots).split(",")]
         serving = min(legs, key=lambda l: l["load"]["50%"]["p99_ms"])
         serving["by_slots"] = {str(l["slots"]): {"saturated_qps": l["saturated_qps"],
                                                  "p99_ms_at_50%": l["load"]["50%"]["p99_ms"]} for l in legs}
-    cpu = None if args.no_cpu else cpu_baseline(idx, q, nprobe_q, K, args)
+    cpu = None if args.no_cpu else cpu_baseline(idx, q, nprobe_q, k_q, args)
 
     line = {
         "metric": "QPS at recall@10>=0.95 (batched GPU search; plain per-query GPU path and p50/p99 reported beside it)",
@@ -374,7 +395,7 @@ def main():
                    "scan_block": {"batch": list(plan), "batch64": list(plan64), "one_query": list(plan1)},
                    "meso_two_phase": {"batch": ws.meso_summary, "batch64": ws64.meso_summary,
                                       "one_query": ws1.meso_summary} if tree else None},
-        f"recall_at_{K}": round(rec, 4),
+        ("recall_at_k_per_query" if per_k else f"recall_at_{K}"): round(rec, 4),
         "recall_at_10": round(state["recall10"], 4),
         "recall_sweep": sweep,
         "parity_vs_oracle": parity,
@@ -396,13 +417,13 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def check_parity(idx, q, nprobe_q, K, n, run, out_ids, out_d, out_i, out_c, torch):
+def check_parity(idx, q, nprobe_q, k_q, n, run, out_ids, out_d, out_i, out_c, torch):
     """GPU results of the first n queries vs the oracle: ids, int32 and float distances bit-exact."""
     from oracle.oracle import OracleIndex
 
     t0 = time.time()
     o = OracleIndex(data=idx)
-    oi, od, oid, oc, _ = o.search(q[:n], K, nprobe_q[:n], threads=min(16, os.cpu_count() or 1))
+    oi, od, oid, oc, _ = o.search(q[:n], k_q[:n], nprobe_q[:n], threads=min(16, os.cpu_count() or 1))
     del o
     run(0, n)
     torch.cuda.synchronize()
@@ -420,24 +441,25 @@ def check_parity(idx, q, nprobe_q, K, n, run, out_ids, out_d, out_i, out_c, torc
     return {"queries": n, "ids_equal": ok_i, "int32_distances_equal": ok_d, "float_distances_equal": ok_f}
 
 
-def serve_leg(_capi, g, q, nprobe_q, cfg, args, plan, slots):
+def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, slots):
     """Continuous batching: saturated QPS, then Poisson arrivals at fractions of it (p50/p99)."""
     K = cfg["k"]
+    budget = int(cfg.get("scratch_gib", 0) * 2**30)  # memory-based admission: slots fit the budget
     sv = _capi.Server(g, max_batch=cfg["batch"], max_k=K, max_nprobe=int(nprobe_q.max()), max_slots=slots,
-                      scan_block=plan)
+                      scratch_budget_bytes=budget, scan_block=plan)
     rng = np.random.default_rng(7)
     n_sat = min(len(q), 8 * cfg["batch"])
     for _ in range(2):  # second pass: warm
-        lat, el = sv.replay(q[:n_sat], K, nprobe_q[:n_sat], np.zeros(n_sat))
+        lat, el = sv.replay(q[:n_sat], k_q[:n_sat], nprobe_q[:n_sat], np.zeros(n_sat))
     sat = n_sat / el
     out = {"saturated_qps": round(sat, 1), "slots": sv.stats()["slots"], "max_batch": cfg["batch"],
-           "scratch_bytes": sv.stats()["scratch_bytes"], "load": {}}
-    for frac in (0.1, 0.5, 0.9):
+           "scratch_bytes": sv.stats()["scratch_bytes"], "scratch_budget_bytes": budget, "load": {}}
+    for frac in cfg.get("loads", (0.1, 0.5, 0.9)):
         rate = frac * sat
         n = int(min(60000, max(2000, rate * 0.4)))
         sel = rng.integers(0, len(q), n)
         arr = np.cumsum(rng.exponential(1.0 / rate, n))
-        lat, el = sv.replay(q[sel], K, nprobe_q[sel], arr)
+        lat, el = sv.replay(q[sel], k_q[sel], nprobe_q[sel], arr)
         out["load"][f"{int(frac * 100)}%"] = {"offered_qps": round(rate, 1), "served_qps": round(n / el, 1),
                                               "p50_ms": round(float(np.percentile(lat, 50)), 4),
                                               "p99_ms": round(float(np.percentile(lat, 99)), 4)}
@@ -450,6 +472,7 @@ def serve_leg(_capi, g, q, nprobe_q, cfg, args, plan, slots):
 
 
 def cpu_baseline(idx, q, nprobe_q, K, args):
+    """K: k for every query, or one k per query."""
     """The oracle (reference semantics, C++) on the host cores, over a bounded query sample."""
     from oracle import oracle as orc
 
@@ -457,16 +480,17 @@ def cpu_baseline(idx, q, nprobe_q, K, args):
     nq = min(len(q), 5000)
     qr = o.preprocess(q[:nq])
     npb = nprobe_q[:nq]
+    kk = np.broadcast_to(np.asarray(K, np.uint32), (len(q),))[:nq]
     t = time.perf_counter()
-    o.search(qr[:20], K, npb[:20], transformed=True, threads=1)
+    o.search(qr[:20], kk[:20], npb[:20], transformed=True, threads=1)
     per_q = (time.perf_counter() - t) / 20
     n1 = int(max(20, min(nq, args.cpu_seconds / max(per_q, 1e-6))))
     t = time.perf_counter()
-    o.search(qr[:n1], K, npb[:n1], transformed=True, threads=1)
+    o.search(qr[:n1], kk[:n1], npb[:n1], transformed=True, threads=1)
     qps1 = n1 / (time.perf_counter() - t)
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1))
     t = time.perf_counter()
-    o.search(qr, K, npb, transformed=True, threads=cores)
+    o.search(qr, kk, npb, transformed=True, threads=cores)
     qpsn = nq / (time.perf_counter() - t)
     log(f"cpu oracle ({orc.isa()}): 1 thread {qps1:.0f} QPS over {n1} queries; {cores} threads {qpsn:.0f} QPS over {nq}")
     return {"value": round(qpsn, 1), "unit": "queries/s", "cores": cores, "kind": "port",
