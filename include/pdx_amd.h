@@ -231,6 +231,51 @@ pdxg_status pdxg_server_get_stats(pdxg_server* server, pdxg_server_stats* out);
 pdxg_status pdxg_server_replay(pdxg_server* server, const float* queries, const uint32_t* k, const uint32_t* nprobe,
                                const double* arrival_s, uint32_t n, float* out_latency_ms, double* elapsed_s);
 
+/* ---------------------------------------------------------------------------------------------
+ * Dense first pass (not in the reference): a copy of an F32 index's data level (every used slot,
+ * normalised and rotated as the index stores it) searched exhaustively by a low-precision MFMA
+ * GEMM (BF16, or FP8 e4m3 with per-tensor scales; FP32 accumulation) over the expansion
+ * ||q||^2 + ||x||^2 - 2 q.x (squared norms accumulated in FP64), keeping k * oversample
+ * candidates per query, which are reranked with the exact FP32 PDX distance -- the one
+ * PDXearch's Start computes for a vector it scans whole (searcher.hpp:218-281): the vertical chain
+ * over the first v dims, then the 64-dim horizontal blocks. Results are (distance, id) ascending,
+ * ties by id. Rows of the copy are addressed by id; the row filter is the bitmap form of the
+ * passing row ids of PDXIndex::FilteredSearch (index.hpp:273-280, 833-845).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct pdxg_dense pdxg_dense;
+enum { PDXG_DENSE_BF16 = 0, PDXG_DENSE_FP8 = 1 };
+
+/* Device buffers; the call is asynchronous on `stream` except when a selection is given (the
+ * passing-row count is read back to size the GEMM). */
+typedef struct {
+    uint32_t num_queries;          /* <= max_queries of the dense copy */
+    const float* queries;          /* num_queries * d */
+    uint32_t queries_transformed;  /* 1: already normalised + rotated */
+    uint32_t k;                    /* results per query */
+    uint32_t oversample;           /* k * oversample (<= 256) candidates are reranked in FP32 */
+    const uint8_t* selection;      /* NULL, or one byte per id (id capacity): 1 = row passes */
+    uint32_t out_stride;           /* >= k */
+    uint32_t* out_ids;             /* num_queries * out_stride */
+    float* out_distances;          /* num_queries * out_stride, may be NULL */
+    uint32_t* out_counts;          /* num_queries */
+} pdxg_dense_search_args;
+
+/* Builds the copy from the index's data level (PDXG_F32 / PDXG_TREE_F32). extra_rows reserves
+ * room for appended rows, whose ids may go up to (largest id + extra_rows). */
+pdxg_status pdxg_dense_create(pdxg_index* index, uint32_t precision, uint64_t extra_rows, uint32_t max_queries,
+                              pdxg_dense** out);
+pdxg_status pdxg_dense_destroy(pdxg_dense* dense);
+pdxg_status pdxg_dense_search(pdxg_dense* dense, const pdxg_dense_search_args* args, void* stream);
+/* Appends n rows (host buffers: n * d floats, n ids); raw rows are normalised and rotated with the
+ * index's rotation first. An id that is already live is replaced. Synchronous. */
+pdxg_status pdxg_dense_append(pdxg_dense* dense, const float* rows, const uint32_t* ids, uint32_t n,
+                              uint32_t rows_transformed);
+/* Deletes rows by id (host buffer); ids without a live row are ignored. Synchronous. */
+pdxg_status pdxg_dense_delete(pdxg_dense* dense, const uint32_t* ids, uint32_t n);
+/* [0] rows (including deleted ones), [1] row capacity, [2] id capacity, [3] device bytes,
+ * [4] rows that entered the last search's GEMM, [5] precision. */
+pdxg_status pdxg_dense_info(const pdxg_dense* dense, uint64_t* out6);
+
 const char* pdxg_last_error(void);
 
 /* Library / device description: "gfx950 <name> <CUs> CUs <LDS> B LDS". */

@@ -63,6 +63,7 @@ def lib():
     L.orc_l2sq16.restype = ctypes.c_float
     L.orc_l2sq16.argtypes = [_f32p, _f32p, ctypes.c_uint32]
     L.orc_l2sq_u8.restype = ctypes.c_uint32
+    L.orc_start_distances.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _f32p]
     L.orc_l2sq_u8.argtypes = [_u8p, _u8p, ctypes.c_uint32]
     L.orc_normalize.argtypes = [_f32p, _f32p, ctypes.c_uint32]
     L.orc_quantize.argtypes = [_f32p, ctypes.c_float, ctypes.c_float, _u8p, ctypes.c_uint32]
@@ -107,6 +108,17 @@ def l2sq16(a: np.ndarray, b: np.ndarray) -> float:
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
     return float(lib().orc_l2sq16(_p(a, _f32p), _p(b, _f32p), a.size))
+
+
+def start_distances(q: np.ndarray, x: np.ndarray, v: int) -> np.ndarray:
+    """F32 distances of the rows of x (row-major, rotated space) to q in the order Start computes
+    them for a vector it scans whole (searcher.hpp:218-281): vertical chain over [0, v), then the
+    64-dim horizontal blocks."""
+    q = np.ascontiguousarray(q, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, q.size)
+    out = np.empty(x.shape[0], np.float32)
+    lib().orc_start_distances(_p(q, _f32p), _p(x, _f32p), x.shape[0], q.size, v, _p(out, _f32p))
+    return out
 
 
 def l2sq_u8(a: np.ndarray, b: np.ndarray) -> int:

@@ -735,6 +735,22 @@ void orc_ratios(uint32_t d, float* out) {
 float orc_l2sq16(const float* a, const float* b, uint32_t d) { return orc::l2sq16(a, b, d); }
 uint32_t orc_l2sq_u8(const uint8_t* a, const uint8_t* b, uint32_t d) { return orc::l2sq_u8(a, b, d); }
 
+// F32 distance of n row-major vectors in the order Start gives a vector it scans whole
+// (searcher.hpp:218-281): the vertical fma chain over dims [0, v), then each 64-dim horizontal
+// block added in turn (distance_computers: vertical, then horizontal in the AVX-512 order).
+void orc_start_distances(const float* q, const float* x, uint32_t n, uint32_t d, uint32_t v, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* row = x + (size_t)i * d;
+        float acc = 0.0f;
+        for (uint32_t j = 0; j < v; ++j) {
+            const float df = q[j] - row[j];
+            acc = std::fmaf(df, df, acc);
+        }
+        for (uint32_t hb = v; hb < d; hb += orc::H_DIM) acc += orc::l2sq16(q + hb, row + hb, orc::H_DIM);
+        out[i] = acc;
+    }
+}
+
 void orc_normalize(const float* x, float* out, uint32_t d) { orc::normalize(x, out, d); }
 
 void orc_quantize(const float* x, float base, float scale, uint8_t* out, uint32_t d) {

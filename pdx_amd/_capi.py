@@ -71,6 +71,15 @@ class SearchArgs(ctypes.Structure):
                 ("out_counts", ctypes.c_void_p), ("out_bytes", ctypes.c_void_p)]
 
 
+class DenseSearchArgs(ctypes.Structure):
+    _fields_ = [("num_queries", ctypes.c_uint32), ("queries", ctypes.c_void_p),
+                ("queries_transformed", ctypes.c_uint32), ("k", ctypes.c_uint32), ("oversample", ctypes.c_uint32),
+                ("selection", ctypes.c_void_p), ("out_stride", ctypes.c_uint32), ("out_ids", ctypes.c_void_p),
+                ("out_distances", ctypes.c_void_p), ("out_counts", ctypes.c_void_p)]
+
+
+PDXG_DENSE_BF16, PDXG_DENSE_FP8 = 0, 1
+
 # every symbol include/pdx_amd.h declares
 EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_index_device_bytes",
            "pdxg_search", "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_workspace_bytes",
@@ -79,7 +88,8 @@ EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster
            "pdxg_workspace_stage_bytes", "pdxg_workspace_meso_times", "pdxg_workspace_meso_pairs",
            "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune",
            "pdxg_workspace_set_meso_summary", "pdxg_workspace_meso_summary", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
-           "pdxg_server_get_stats", "pdxg_server_replay"]
+           "pdxg_server_get_stats", "pdxg_server_replay", "pdxg_dense_create", "pdxg_dense_destroy",
+           "pdxg_dense_search", "pdxg_dense_append", "pdxg_dense_delete", "pdxg_dense_info"]
 
 _lib = None
 
@@ -130,6 +140,15 @@ def lib():
     L.pdxg_server_poll.argtypes = [vp, u32, ctypes.c_int, u64p, u32, u32p, f32p, u32p, f32p, u32p]
     L.pdxg_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     L.pdxg_server_replay.argtypes = [vp, f32p, u32p, u32p, dblp, u32, f32p, dblp]
+    L.pdxg_dense_create.argtypes = [vp, u32, ctypes.c_uint64, u32, ctypes.POINTER(vp)]
+    L.pdxg_dense_destroy.argtypes = [vp]
+    L.pdxg_dense_search.argtypes = [vp, ctypes.POINTER(DenseSearchArgs), vp]
+    L.pdxg_dense_append.argtypes = [vp, f32p, u32p, u32, u32]
+    L.pdxg_dense_delete.argtypes = [vp, u32p, u32]
+    L.pdxg_dense_info.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    for name in ("pdxg_dense_create", "pdxg_dense_destroy", "pdxg_dense_search", "pdxg_dense_append",
+                 "pdxg_dense_delete", "pdxg_dense_info"):
+        getattr(L, name).restype = ctypes.c_int
     for name in ("pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_search",
                  "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device", "pdxg_preprocess",
                  "pdxg_filtered_search", "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing",
@@ -351,6 +370,54 @@ class Workspace:
                        nprobe_all, k_max, nprobe_max, out_stride, out_ids or None, out_dists or None,
                        out_idists or None, out_counts or None, out_bytes or None)
         check(lib().pdxg_search_device(self.index.handle, self._h, ctypes.byref(a), ctypes.c_void_p(stream or 0)))
+
+
+class Dense:
+    """Dense low-precision first pass + exact FP32 rerank over a copy of an F32 index (pdxg_dense_*)."""
+
+    def __init__(self, index: GpuIndex, precision: int = PDXG_DENSE_BF16, extra_rows: int = 0,
+                 max_queries: int = 1024):
+        self.index = index
+        self.precision = precision
+        h = ctypes.c_void_p()
+        check(lib().pdxg_dense_create(index.handle, precision, extra_rows, max_queries, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pdxg_dense_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        out = np.zeros(6, np.uint64)
+        check(lib().pdxg_dense_info(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return {"rows": int(out[0]), "capacity": int(out[1]), "id_capacity": int(out[2]),
+                "device_bytes": int(out[3]), "gemm_rows": int(out[4]), "precision": int(out[5])}
+
+    def search(self, queries_ptr: int, nq: int, k: int, oversample: int, out_stride: int, out_ids: int,
+               out_dists: int, out_counts: int, selection_ptr: int = 0, transformed: bool = False,
+               stream: int = 0) -> None:
+        """Search on device pointers (ints; 0 = NULL) on `stream` (hipStream_t)."""
+        a = DenseSearchArgs(nq, queries_ptr or None, int(transformed), k, oversample, selection_ptr or None,
+                            out_stride, out_ids or None, out_dists or None, out_counts or None)
+        check(lib().pdxg_dense_search(self._h, ctypes.byref(a), ctypes.c_void_p(stream or 0)))
+
+    def append(self, rows: np.ndarray, ids: np.ndarray, transformed: bool = False) -> None:
+        r = np.ascontiguousarray(rows, np.float32).reshape(-1, self.index.data.d)
+        i = np.ascontiguousarray(ids, np.uint32).reshape(-1)
+        assert r.shape[0] == i.size
+        check(lib().pdxg_dense_append(self._h, _ptr(r, ctypes.POINTER(ctypes.c_float)),
+                                      _ptr(i, ctypes.POINTER(ctypes.c_uint32)), i.size, int(transformed)))
+
+    def delete(self, ids: np.ndarray) -> None:
+        i = np.ascontiguousarray(ids, np.uint32).reshape(-1)
+        check(lib().pdxg_dense_delete(self._h, _ptr(i, ctypes.POINTER(ctypes.c_uint32)), i.size))
 
 
 class Server:

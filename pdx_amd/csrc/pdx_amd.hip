@@ -1185,3 +1185,4 @@ pdxg_status pdxg_filtered_search(pdxg_index* ix, const float* query, uint32_t tr
 }  // extern "C"
 
 #include "pdx_server.inc"
+#include "pdx_dense.inc"

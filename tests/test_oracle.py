@@ -205,6 +205,26 @@ def test_returned_distances_are_full_distances(kind):
                 assert abs(rd[t, r] - e) <= 1e-5 * max(e, 1e-6)
 
 
+@pytest.mark.parametrize("d", [48, 128, 130, 384])
+def test_start_distances_equal_the_search_of_a_single_cluster(d):
+    """With one cluster the search scans every vector through Start, so its distances are the
+    Start-order distances: start_distances must reproduce them bit for bit (ids in the same
+    (distance, id) order), and stay within FP32 rounding of the FP64 distance."""
+    path, x, q = index_file("pdx_f32", 300, d, 1)
+    o = OracleIndex(path)
+    idx, vecs, ids = _decoded(path)
+    qr = o.preprocess(q)
+    rid, rd, _, cnt, _ = o.search(qr, 300, 1, transformed=True)
+    for t in range(len(q)):
+        sd = orc.start_distances(qr[t], vecs, idx.l1.v)
+        order = np.lexsort((ids, sd))
+        assert int(cnt[t]) == 300
+        np.testing.assert_array_equal(rid[t], ids[order])
+        np.testing.assert_array_equal(rd[t].view(np.uint32), sd[order].view(np.uint32))
+        e = ((vecs.astype(np.float64) - qr[t].astype(np.float64)) ** 2).sum(1)
+        np.testing.assert_allclose(sd, e, rtol=1e-5, atol=1e-6)
+
+
 @pytest.mark.parametrize("kind", ["pdx_f32", "pdx_u8", "pdx_tree_f32", "pdx_tree_u8"])
 def test_exhaustive_recall_and_nprobe_equivalences(kind):
     path, x, q = index_file(kind, 3000, 128, 24)
