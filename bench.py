@@ -7,6 +7,12 @@ API does by default, 17,889 lists / 133 meso clusters, k = 20, batches of 4,096 
 per-query nprobe drawn uniformly from [8, 128]; results are checked bit-exact against the oracle
 in the run.
 
+--config c5: dense low-precision first pass (BF16 and FP8 e4m3 GEMM, FP32 accumulation) with a 4x
+oversampled exact FP32 rerank over a copy of an F32 IVF index. N = 2,000,000 unit-norm vectors of
+d = 3,072 (2,048-component mixture, noise norm 0.1), batches of 2,048 queries, k = 10, a seeded 10%
+row filter and 1% of the rows appended and deleted between the timed batches; the error of the
+returned distances against FP64 is reported.
+
 --config c2: IVF float32 with ADSampling. N = 1,000,000 unit-norm vectors of d = 768 from a
 1,024-component Gaussian mixture (centres ~ N(0, I) normalised, noise norm 0.15, rows
 renormalised), 1,000 IVF lists, 10,000 queries from the mixture, k = 10, nprobe = 32, L2.
@@ -43,7 +49,12 @@ CONFIGS = {
     "c4": dict(n=10_000_000, d=1536, centers=8192, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
                normalize=True, nq=8192, batch=4096, k=100, k_choices=(10, 20, 100), nprobe=0, nprobe_range=None,
                nprobe_choices=(16, 64, 256), gt=256, loads=(0.1, 0.25, 0.5, 0.75, 0.9, 1.0), scratch_gib=24),
+    # dense low-precision first pass + FP32 rerank with a row filter and interleaved updates
+    "c5": dict(n=2_000_000, d=3072, centers=2048, sigma=0.1, unit=True, kind="pdx_f32", nlist=0, n_meso=0,
+               normalize=False, nq=8192, batch=2048, k=10, nprobe=0, nprobe_range=None, gt=512, dense=True,
+               oversample=4, filter=0.10, update_frac=0.01, precisions=("bf16", "fp8")),
 }
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp8": 5000.0}  # MI355X dense (no sparsity)
 
 
 def log(*a):
@@ -76,8 +87,10 @@ def mixture(n, d, n_centers, sigma, seed, n_queries, unit=True, per_coord=False)
     return x, q.astype(np.float32)
 
 
-def ground_truth(x, q, k, normalize, torch, dev, chunk=1 << 20):
-    """Exact top-k ids by L2 in float64 on the GPU (rows and queries normalised if the index is)."""
+def ground_truth(x, q, k, normalize, torch, dev, chunk=None, mask=None):
+    """Exact top-k ids by L2 in float64 on the GPU (rows and queries normalised if the index is);
+    rows with mask False are excluded."""
+    chunk = chunk or min(1 << 20, (6 << 30) // (8 * x.shape[1]))  # about 6 GiB of float64 rows per chunk
     qt = torch.from_numpy(q).to(dev, torch.float64)
     if normalize:
         qt = qt / torch.linalg.vector_norm(qt, dim=1, keepdim=True)
@@ -89,6 +102,9 @@ def ground_truth(x, q, k, normalize, torch, dev, chunk=1 << 20):
         if normalize:
             xt = xt / torch.linalg.vector_norm(xt, dim=1, keepdim=True)
         dd = (xt * xt).sum(1)[None, :] - 2.0 * (qt @ xt.T) + qn[:, None]
+        if mask is not None:
+            mt = torch.from_numpy(~mask[s:s + chunk]).to(dev)
+            dd[:, mt] = float("inf")
         vd, vi = torch.topk(dd, min(k, dd.shape[1]), dim=1, largest=False)
         cd = torch.cat([best_d, vd], 1)
         ci = torch.cat([best_i, vi + s], 1)
@@ -147,6 +163,8 @@ def main():
 
     dev = torch.device("cuda", 0)
     log("device:", _capi.device_info(0))
+    if cfg.get("dense"):
+        return bench_dense(args, cfg, torch, _capi, builder, dev)
     t0 = time.time()
     x, q = mixture(cfg["n"], cfg["d"], cfg["centers"], cfg["sigma"], args.seed, cfg["nq"], cfg["unit"],
                    args.sigma_per_coord)
@@ -415,6 +433,234 @@ def main():
         "build_s": round(build_s, 1),
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_dense(args, cfg, torch, _capi, builder, dev):
+    """Dense first pass (BF16 / FP8 e4m3 GEMM, FP32 accumulation) + k*oversample FP32 rerank over
+    a copy of an F32 index, with a seeded row filter and append/delete of update_frac of the rows
+    interleaved with the timed batches. One JSON line; value = queries / (search + update time) of
+    the fastest precision at recall@10 >= the target."""
+    from pdx_amd import pdxfile
+
+    n, d, B, K, NQ = cfg["n"], cfg["d"], cfg["batch"], cfg["k"], cfg["nq"]
+    OS = cfg["oversample"]
+    n_up = int(round(n * cfg["update_frac"]))
+    t0 = time.time()
+    x_all, q = mixture(n + n_up, d, cfg["centers"], cfg["sigma"], args.seed, NQ, cfg["unit"], args.sigma_per_coord)
+    x, x_new = x_all[:n], x_all[n:]
+    log(f"data {x.shape} (+{n_up} rows to append) in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    if args.index_cache and os.path.exists(args.index_cache):
+        idx = pdxfile.load(args.index_cache)
+        log(f"index loaded from {args.index_cache}")
+    else:
+        idx = builder.build_index(x, cfg["kind"], num_clusters=cfg["nlist"], normalize=cfg["normalize"],
+                                  distance_metric=args.metric, seed=42, device=0)
+        if args.index_cache:
+            pdxfile.save(idx, args.index_cache)
+    build_s = time.time() - t0
+    log(f"index {cfg['kind']} with {idx.l1.nc} lists in {build_s:.1f}s")
+    g = _capi.GpuIndex(idx)
+    rng = np.random.default_rng(args.seed + 2)
+    sel = (rng.random(n + n_up) < cfg["filter"]).astype(np.uint8)  # row filter by id, appended ids included
+    sel_d = torch.from_numpy(sel).to(dev)
+    gtq = min(cfg["gt"], NQ)
+    t0 = time.time()
+    gt = ground_truth(x, q[:gtq], K, cfg["normalize"], torch, dev, mask=sel[:n] == 1)
+    log(f"fp64 filtered ground truth for {gtq} queries in {time.time() - t0:.1f}s")
+    qd = torch.from_numpy(q).to(dev)
+    out_ids = torch.zeros((B, K), dtype=torch.int32, device=dev)
+    out_d = torch.zeros((B, K), dtype=torch.float32, device=dev)
+    out_c = torch.zeros(B, dtype=torch.int32, device=dev)
+    sp = torch.cuda.current_stream().cuda_stream
+    xq = q.astype(np.float64)
+    if cfg["normalize"]:
+        xq /= np.linalg.norm(xq, axis=1, keepdims=True)
+
+    def search(dn, s, m, filtered=True):
+        dn.search(qd[s:s + m].data_ptr(), m, K, OS, K, out_ids.data_ptr(), out_d.data_ptr(), out_c.data_ptr(),
+                  selection_ptr=sel_d.data_ptr() if filtered else 0, stream=sp)
+
+    def measure(dn, rows, live_mask, label):
+        """recall@k vs the FP64 ground truth of the live passing rows, and the error of the returned
+        FP32 distances against FP64 distances of the same (unrotated) rows."""
+        g_t = gt if label == "before" else ground_truth(rows, q[:gtq], K, cfg["normalize"], torch, dev,
+                                                        mask=live_mask)
+        ids = np.zeros((gtq, K), np.uint32)
+        dst = np.zeros((gtq, K), np.float32)
+        cnt = np.zeros(gtq, np.uint32)
+        for s0 in range(0, gtq, B):
+            m = min(B, gtq - s0)
+            search(dn, s0, m)
+            torch.cuda.synchronize()
+            ids[s0:s0 + m] = out_ids[:m].cpu().numpy().view(np.uint32)
+            dst[s0:s0 + m] = out_d[:m].cpu().numpy()
+            cnt[s0:s0 + m] = out_c[:m].cpu().numpy()
+        rec = recall(ids, cnt, g_t, K)
+        errs = []
+        for t in range(gtq):
+            r = rows[ids[t, :cnt[t]]].astype(np.float64)
+            if cfg["normalize"]:
+                r /= np.linalg.norm(r, axis=1, keepdims=True)
+            e = ((r - xq[t]) ** 2).sum(1)
+            errs.append(np.abs(dst[t, :cnt[t]].astype(np.float64) - e) / np.maximum(e, 1e-30))
+        errs = np.concatenate(errs)
+        return rec, {"max_rel": float(errs.max()), "mean_rel": float(errs.mean())}
+
+    results = {}
+    for pname in cfg["precisions"]:
+        prec = _capi.PDXG_DENSE_BF16 if pname == "bf16" else _capi.PDXG_DENSE_FP8
+        dn = _capi.Dense(g, prec, extra_rows=n_up, max_queries=B)
+        log(f"{pname}: dense copy {dn.info()['device_bytes'] / 2**30:.1f} GiB")
+        rec0, err0 = measure(dn, x, sel[:n] == 1, "before")
+        log(f"{pname}: recall@{K} {rec0:.4f}; distance error vs fp64 max {err0['max_rel']:.2e} "
+            f"mean {err0['mean_rel']:.2e}")
+        # unfiltered batch (the full-size GEMM) for reference
+        for _ in range(2):
+            search(dn, 0, B, filtered=False)
+        torch.cuda.synchronize()
+        dn.enable_timing(True)
+        t = time.perf_counter()
+        reps = 3
+        st_unf = np.zeros(4)
+        for i in range(reps):
+            search(dn, 0, B, filtered=False)
+            st_unf += dn.stage_times()
+        torch.cuda.synchronize()
+        unf_ms = (time.perf_counter() - t) * 1e3 / reps
+        st_unf /= reps
+        unf_rows = dn.info()["gemm_rows"]
+        # timed: filtered batches with append/delete interleaved (update_frac of the rows in total)
+        nb = max(1, NQ // B)
+        for i in range(args.warmup):
+            search(dn, (i % nb) * B, B)
+        torch.cuda.synchronize()
+        live = np.zeros(n + n_up, bool)
+        live[:n] = True
+        per = [n_up // args.steps + (1 if i < n_up % args.steps else 0) for i in range(args.steps)]
+        urng = np.random.default_rng(args.seed + 3)
+        nxt = 0
+        t_search = t_upd = 0.0
+        stage = np.zeros(4)
+        gemm_rows = 0
+        for i in range(args.steps):
+            t = time.perf_counter()
+            search(dn, (i % nb) * B, B)
+            torch.cuda.synchronize()
+            t_search += time.perf_counter() - t
+            stage += dn.stage_times()
+            gemm_rows += dn.info()["gemm_rows"]
+            t = time.perf_counter()
+            a = per[i]
+            if a:
+                new_ids = np.arange(n + nxt, n + nxt + a, dtype=np.uint32)
+                dn.append(x_new[nxt:nxt + a], new_ids)
+                live[new_ids] = True
+                nxt += a
+                cand = np.nonzero(live[:n])[0]
+                gone = urng.choice(cand, a, replace=False).astype(np.uint32)
+                dn.delete(gone)
+                live[gone] = False
+            t_upd += time.perf_counter() - t
+        stage /= args.steps
+        gemm_rows /= args.steps
+        qps_search = B * args.steps / t_search
+        qps_total = B * args.steps / (t_search + t_upd)
+        dn.enable_timing(False)
+        rec1, err1 = measure(dn, x_all, live & (sel == 1), "after")
+        flops = 2.0 * B * gemm_rows * d
+        tf = flops / (stage[1] * 1e-3) / 1e12
+        sel_bytes = B * gemm_rows * 4.0
+        sel_gbs = sel_bytes / (stage[2] * 1e-3) / 1e9
+        unf_tf = 2.0 * B * unf_rows * d / (st_unf[1] * 1e-3) / 1e12
+        log(f"{pname}: {qps_total:.0f} QPS with updates ({qps_search:.0f} search only); stage ms "
+            f"filter/gemm/select/rerank = {stage.round(4)}; GEMM {tf:.0f} TFLOP/s over {gemm_rows:.0f} rows; "
+            f"unfiltered batch {unf_ms:.2f} ms, GEMM {unf_tf:.0f} TFLOP/s; recall after updates {rec1:.4f}")
+        results[pname] = {
+            "qps": round(qps_total, 1), "qps_search_only": round(qps_search, 1),
+            "ms_per_step": round((t_search + t_upd) * 1e3 / args.steps, 4),
+            "update_ms_per_step": round(t_upd * 1e3 / args.steps, 4),
+            "recall_at_10_before_updates": round(rec0, 4), "recall_at_10_after_updates": round(rec1, 4),
+            "distance_error_vs_fp64": {"before": err0, "after": err1},
+            "stage_ms": {"queries_and_filter": round(float(stage[0]), 4), "gemm": round(float(stage[1]), 4),
+                         "select": round(float(stage[2]), 4), "rerank": round(float(stage[3]), 4)},
+            "gemm_rows_per_batch": int(gemm_rows),
+            "kernels": {
+                "gemm": {"bound": "mfma", "kernel": f"hipBLASLt {pname} GEMM, FP32 accumulation",
+                         "achieved": round(tf, 1), "peak": MFMA_PEAK_TFLOPS[pname], "unit": "TFLOP/s",
+                         "frac": round(tf / MFMA_PEAK_TFLOPS[pname], 4), "ms": round(float(stage[1]), 4),
+                         "flops_per_launch": int(flops)},
+                "select": {"bound": "hbm", "kernel": "k_dense_select + k_dense_merge (top-R of the GEMM output)",
+                           "achieved": round(sel_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(sel_gbs / HBM_PEAK_GBS, 4), "ms": round(float(stage[2]), 4),
+                           "algorithmic_bytes_per_launch": int(sel_bytes)},
+            },
+            "unfiltered_batch": {"ms": round(unf_ms, 4), "gemm_rows": int(unf_rows),
+                                 "gemm_ms": round(float(st_unf[1]), 4), "gemm_tflops": round(unf_tf, 1),
+                                 "stage_ms": [round(float(v), 4) for v in st_unf]},
+        }
+        dn.close()
+        torch.cuda.empty_cache()
+
+    ok = {p: r for p, r in results.items() if r["recall_at_10_before_updates"] >= args.target_recall}
+    best = max(ok or results, key=lambda p: (ok or results)[p]["qps"])
+    workload = (f"dense first pass over {cfg['kind']}, N={n}, d={d}, k={K}, oversample={OS}, batch={B}, "
+                f"{int(cfg['filter'] * 100)}% row filter, {cfg['update_frac'] * 100:g}% append/delete")
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            tr = json.load(f).get(workload, {})
+    except (OSError, ValueError):
+        tr = {}
+    for p, r in results.items():
+        for kname, kr in r["kernels"].items():
+            kr["traffic"] = (tr.get(f"{p}_{kname}") or {}).get("traffic_bytes_per_launch")
+    cpu = None if args.no_cpu else cpu_baseline_dense(idx, q, sel, K, args)
+    line = {
+        "metric": "QPS at recall@10>=0.95 (dense low-precision first pass + exact FP32 rerank, row filter, "
+                  "interleaved append/delete)",
+        "value": results[best]["qps"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": results[best]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{best} GEMM (fp32 accumulation) + f32 rerank",
+        "data": "synthetic (seeded Gaussian mixture; index built in-process)",
+        "config": {"workload": workload, "global_batch": B, "parallelism": "single GPU", "precision": best},
+        "recall_at_10": results[best]["recall_at_10_before_updates"],
+        "precisions": results,
+        "roofline": dict(results[best]["kernels"]["gemm"], stage="gemm"),
+        "cpu_baseline": cpu,
+        "build_s": round(build_s, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_dense(idx, q, sel, K, args):
+    """The oracle's filtered search over every list (the exhaustive filtered PDX search of the
+    reference) on one host core, over a bounded query sample."""
+    from oracle import oracle as orc
+
+    o = orc.OracleIndex(data=idx)
+    L = idx.l1
+    slot = np.zeros(int(sum(c.cap for c in L.clusters)), np.uint8)
+    off = 0
+    for c in L.clusters:
+        slot[off:off + c.n_used] = sel[c.ids[:c.n_used]]
+        off += c.cap
+    qr = o.preprocess(q[:200])
+    t = time.perf_counter()
+    n1 = 0
+    while n1 < len(qr) and (n1 < 3 or time.perf_counter() - t < args.cpu_seconds):
+        o.filtered_search(qr[n1], K, 0, slot, transformed=True)
+        n1 += 1
+    qps = n1 / (time.perf_counter() - t)
+    log(f"cpu oracle ({orc.isa()}): exhaustive filtered search {qps:.2f} QPS over {n1} queries on 1 thread")
+    return {"value": round(qps, 3), "unit": "queries/s", "cores": 1, "kind": "port", "isa": orc.isa(),
+            "sample": f"{n1} pre-rotated queries, filtered search over every list (nprobe = all), 1 thread"}
 
 
 def check_parity(idx, q, nprobe_q, k_q, n, run, out_ids, out_d, out_i, out_c, torch):

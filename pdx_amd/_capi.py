@@ -89,7 +89,8 @@ EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster
            "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune",
            "pdxg_workspace_set_meso_summary", "pdxg_workspace_meso_summary", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
            "pdxg_server_get_stats", "pdxg_server_replay", "pdxg_dense_create", "pdxg_dense_destroy",
-           "pdxg_dense_search", "pdxg_dense_append", "pdxg_dense_delete", "pdxg_dense_info"]
+           "pdxg_dense_search", "pdxg_dense_append", "pdxg_dense_delete", "pdxg_dense_info",
+           "pdxg_dense_enable_timing", "pdxg_dense_stage_times"]
 
 _lib = None
 
@@ -146,8 +147,10 @@ def lib():
     L.pdxg_dense_append.argtypes = [vp, f32p, u32p, u32, u32]
     L.pdxg_dense_delete.argtypes = [vp, u32p, u32]
     L.pdxg_dense_info.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.pdxg_dense_enable_timing.argtypes = [vp, ctypes.c_int]
+    L.pdxg_dense_stage_times.argtypes = [vp, f32p]
     for name in ("pdxg_dense_create", "pdxg_dense_destroy", "pdxg_dense_search", "pdxg_dense_append",
-                 "pdxg_dense_delete", "pdxg_dense_info"):
+                 "pdxg_dense_delete", "pdxg_dense_info", "pdxg_dense_enable_timing", "pdxg_dense_stage_times"):
         getattr(L, name).restype = ctypes.c_int
     for name in ("pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_search",
                  "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device", "pdxg_preprocess",
@@ -399,6 +402,15 @@ class Dense:
         check(lib().pdxg_dense_info(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
         return {"rows": int(out[0]), "capacity": int(out[1]), "id_capacity": int(out[2]),
                 "device_bytes": int(out[3]), "gemm_rows": int(out[4]), "precision": int(out[5])}
+
+    def enable_timing(self, on: bool = True) -> None:
+        check(lib().pdxg_dense_enable_timing(self._h, int(on)))
+
+    def stage_times(self):
+        """Milliseconds of [queries + filter, GEMM, top-R select, FP32 rerank] of the last search."""
+        out = np.zeros(4, np.float32)
+        check(lib().pdxg_dense_stage_times(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
 
     def search(self, queries_ptr: int, nq: int, k: int, oversample: int, out_stride: int, out_ids: int,
                out_dists: int, out_counts: int, selection_ptr: int = 0, transformed: bool = False,

@@ -529,6 +529,8 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     const size_t items = pairs / kMesoQ + L.nc;
     const size_t wgs = (items + kMesoWaves - 1) / kMesoWaves;
     m.wg_per_xcd = (uint32_t)((wgs + 7) / 8);
+    require(8ull * m.wg_per_xcd * kMesoWaves * kWave < (1ull << 32), "too many meso work items for one dispatch",
+            PDXG_ERR_UNSUPPORTED);
     hipLaunchKernelGGL(k_meso_summary, dim3(8 * m.wg_per_xcd), dim3(kMesoWaves * kWave), 0, st, m);
     HIP_CHECK(hipGetLastError());
 }

@@ -95,14 +95,17 @@ __global__ void k_dense_max_id(const uint32_t* ids, unsigned long long nr, uint3
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-// Low-precision copy of rows [r0, r0 + nr): BF16 (unscaled) or FP8 e4m3 scaled by *sx.
+// Low-precision copy of rows [r0, r0 + nr): BF16 (unscaled) or FP8 e4m3 scaled by *sx. Grid-stride
+// (a dispatch holds at most 2^32 work-items per dimension; a 2M x 3072 copy has 6.1e9 elements).
 __global__ void k_dense_convert(const float* __restrict__ x32, void* xlp, unsigned long long r0, unsigned long long nr,
                                 uint32_t d, uint32_t prec, const float* sx) {
-    const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nr * d) return;
-    const unsigned long long g = r0 * d + e;
-    if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(xlp)[g] = __float2bfloat16(x32[g]);
-    else reinterpret_cast<__hip_fp8_storage_t*>(xlp)[g] = __hip_cvt_float_to_fp8(x32[g] * *sx, __HIP_SATFINITE, __HIP_E4M3);
+    const unsigned long long total = nr * d, step = (unsigned long long)gridDim.x * blockDim.x;
+    const float scale = prec == kDenseBF16 ? 1.0f : *sx;
+    for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += step) {
+        const unsigned long long g = r0 * d + e;
+        if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(xlp)[g] = __float2bfloat16(x32[g]);
+        else reinterpret_cast<__hip_fp8_storage_t*>(xlp)[g] = __hip_cvt_float_to_fp8(x32[g] * scale, __HIP_SATFINITE, __HIP_E4M3);
+    }
 }
 
 // FP8 scales from a largest magnitude: s = kFp8Range / m (values are multiplied by s before the
@@ -203,16 +206,18 @@ __global__ __launch_bounds__(256) void k_dense_compact(const uint8_t* live, cons
 }
 
 // Packed low-precision rows: dst[p] = src[rowlist[p]] for p < n (row_bytes a multiple of 16).
+// Grid-stride over the 16-byte words.
 __global__ __launch_bounds__(256) void k_dense_gather(const uint8_t* __restrict__ src, const uint32_t* rowlist,
                                                       unsigned long long n, uint32_t row_bytes,
                                                       uint8_t* __restrict__ dst) {
     const uint32_t per_row = row_bytes / 16;
-    const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned long long p = e / per_row;
-    if (p >= n) return;
-    const uint32_t w = static_cast<uint32_t>(e % per_row);
-    const uint4* s = reinterpret_cast<const uint4*>(src + (size_t)rowlist[p] * row_bytes);
-    reinterpret_cast<uint4*>(dst + p * row_bytes)[w] = s[w];
+    const unsigned long long total = n * per_row, step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += step) {
+        const unsigned long long p = e / per_row;
+        const uint32_t w = static_cast<uint32_t>(e % per_row);
+        const uint4* s = reinterpret_cast<const uint4*>(src + (size_t)rowlist[p] * row_bytes);
+        reinterpret_cast<uint4*>(dst + p * row_bytes)[w] = s[w];
+    }
 }
 
 // Unfiltered bias: ||x||^2 for live rows, +inf for deleted ones.
