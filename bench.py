@@ -52,9 +52,9 @@ CONFIGS = {
     # dense low-precision first pass + FP32 rerank with a row filter and interleaved updates
     "c5": dict(n=2_000_000, d=3072, centers=2048, sigma=0.1, unit=True, kind="pdx_f32", nlist=0, n_meso=0,
                normalize=False, nq=8192, batch=2048, k=10, nprobe=0, nprobe_range=None, gt=512, dense=True,
-               oversample=4, filter=0.10, update_frac=0.01, precisions=("bf16", "fp8")),
+               oversample=4, filter=0.10, update_frac=0.01, precisions=("bf16", "fp8", "fp8_direct")),
 }
-MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp8": 5000.0}  # MI355X dense (no sparsity)
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp8": 5000.0, "fp8_direct": 5000.0}  # MI355X dense (no sparsity)
 
 
 def log(*a):
@@ -384,6 +384,49 @@ def main():
     lat1 = np.array(lat1) * 1e3
     lat64 = np.array(lat64[2:]) * 1e3
     plain_qps = 1e3 / lat1.mean()
+
+    # the same searches captured in HIP graphs (pdxg_graph_*): fixed device buffers that each
+    # query (batch) is copied into, then one graph launch
+    def graph_latency(w, m, reps):
+        qb = torch.empty((m, cfg["d"]), dtype=torch.float32, device=dev)
+        kb = kd[:m].clone()
+        nb_ = state["np_dev"][:m].clone()
+        gr = w.capture(qb.data_ptr(), m, kb.data_ptr() if per_k else 0, nb_.data_ptr(), K, state["np_max"], K,
+                       out_ids.data_ptr(), out_d.data_ptr(), out_i.data_ptr(), out_c.data_ptr(), k_all=K)
+        lat = []
+        for i in range(reps + 5):
+            s0 = (i * m) % (NQ - m + 1)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            qb.copy_(qd[s0:s0 + m])
+            kb.copy_(kd[s0:s0 + m])
+            nb_.copy_(state["np_dev"][s0:s0 + m])
+            gr.launch(sp)
+            torch.cuda.synchronize()
+            if i >= 5:
+                lat.append(time.perf_counter() - t)
+        nodes = gr.nodes
+        gr.close()
+        return np.array(lat) * 1e3, nodes
+
+    # service time of one batch by size (what a serving loop's batch costs)
+    svc = {}
+    for m in (1, 16, 64, 256, 1024, B):
+        w = ws1 if m == 1 else (ws64 if m <= 64 else state["ws"])
+        tt = []
+        for i in range(8):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            run((i * m) % (NQ - m + 1), m, w=w)
+            torch.cuda.synchronize()
+            tt.append(time.perf_counter() - t)
+        svc[str(m)] = round(float(np.median(tt[2:])) * 1e3, 4)
+    log(f"batch service time ms by size: {svc}")
+
+    lat1g, nodes1 = graph_latency(ws1, 1, args.plain_queries)
+    lat64g, _ = graph_latency(ws64, 64, 60)
+    log(f"graph-captured: one query p50 {np.percentile(lat1g, 50):.3f} ms p99 {np.percentile(lat1g, 99):.3f} ms "
+        f"({nodes1} nodes); batch-64 p50 {np.percentile(lat64g, 50):.3f} ms")
     log(f"plain per-query: {plain_qps:.0f} QPS p50 {np.percentile(lat1, 50):.3f} ms p99 {np.percentile(lat1, 99):.3f} ms; "
         f"batch-64 p50 {np.percentile(lat64, 50):.3f} ms")
 
@@ -423,7 +466,11 @@ def main():
                        "plain_p99": round(float(np.percentile(lat1, 99)), 4),
                        "batch64_p50": round(float(np.percentile(lat64, 50)), 4),
                        "batch64_p99": round(float(np.percentile(lat64, 99)), 4),
+                       "graph_one_query_p50": round(float(np.percentile(lat1g, 50)), 4),
+                       "graph_one_query_p99": round(float(np.percentile(lat1g, 99)), 4),
+                       "graph_batch64_p50": round(float(np.percentile(lat64g, 50)), 4),
                        f"batch{B}_mean": round(ms_per_step, 4)},
+        "batch_service_ms": svc,
         "stage_ms": {"preprocess": round(float(stage[0]), 4), "probe": round(float(stage[1]), 4),
                      "meso_scan": round(float(stage[2]), 4), "scan": round(float(stage[3]), 4)},
         "roofline": dict(roofs[dom_key], stage=dom_key),
@@ -509,7 +556,9 @@ def bench_dense(args, cfg, torch, _capi, builder, dev):
 
     results = {}
     for pname in cfg["precisions"]:
-        prec = _capi.PDXG_DENSE_BF16 if pname == "bf16" else _capi.PDXG_DENSE_FP8
+        # "fp8_direct": FP8 of the rows themselves (no residual coding), for comparison
+        prec = {"bf16": _capi.PDXG_DENSE_BF16, "fp8": _capi.PDXG_DENSE_FP8,
+                "fp8_direct": _capi.PDXG_DENSE_FP8 | _capi.PDXG_DENSE_NO_RESIDUAL}[pname]
         dn = _capi.Dense(g, prec, extra_rows=n_up, max_queries=B)
         log(f"{pname}: dense copy {dn.info()['device_bytes'] / 2**30:.1f} GiB")
         rec0, err0 = measure(dn, x, sel[:n] == 1, "before")
@@ -694,6 +743,9 @@ def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, slots):
     sv = _capi.Server(g, max_batch=cfg["batch"], max_k=K, max_nprobe=int(nprobe_q.max()), max_slots=slots,
                       scratch_budget_bytes=budget, scan_block=plan)
     rng = np.random.default_rng(7)
+    # scan plans per batch size (the batches under load are far smaller than the saturated ones)
+    ns = min(len(q), cfg["batch"])
+    ladder = sv.autotune(q[:ns], k_q[:ns], nprobe_q[:ns])
     n_sat = min(len(q), 8 * cfg["batch"])
     for _ in range(2):  # second pass: warm
         lat, el = sv.replay(q[:n_sat], k_q[:n_sat], nprobe_q[:n_sat], np.zeros(n_sat))
@@ -713,6 +765,7 @@ def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, slots):
             f"p99 {np.percentile(lat, 99):.3f} ms")
     st = sv.stats()
     out["mean_batch"] = round(st["completed"] / max(1, st["batches"]), 1)
+    out["plans"] = [list(p) for p in ladder]
     sv.close()
     return out
 

@@ -156,6 +156,16 @@ pdxg_status pdxg_workspace_set_meso_summary(pdxg_workspace* ws, int enable);
 int pdxg_workspace_meso_summary(const pdxg_workspace* ws);
 pdxg_status pdxg_workspace_autotune(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,
                                     void* stream);
+/* Order of the per-query scans (a query's clusters always run in order on one workgroup, as the
+ * reference's threshold chain requires): 0 = one workgroup per query in query order;
+ * 1 (default) = longest first: the queries sorted on device by descending data-level visit count,
+ * so the longest cluster walks start first and short ones fill in behind them; 2 = longest first
+ * through a work queue: a persistent grid of as many workgroups as fit on the device at once takes
+ * the sorted queries with an atomic counter until none is left. Results do not depend on it; the
+ * autotuner keeps the fastest. pdxg_workspace_order returns the current mode. Not in the
+ * reference. */
+pdxg_status pdxg_workspace_set_order(pdxg_workspace* ws, int mode);
+int pdxg_workspace_order(const pdxg_workspace* ws);
 
 /* Asynchronous batched search on `stream` (a hipStream_t, NULL = default stream) with device
  * buffers. The workspace must not be shared by calls that overlap in time. */
@@ -225,6 +235,17 @@ pdxg_status pdxg_server_poll(pdxg_server* server, uint32_t max_results, int time
                              uint32_t out_stride, uint32_t* out_ids, float* out_distances, uint32_t* out_counts,
                              float* out_latency_ms, uint32_t* num_results);
 pdxg_status pdxg_server_get_stats(pdxg_server* server, pdxg_server_stats* out);
+/* Scan plans per batch size: runs pdxg_workspace_autotune on one slot for batches of 1, 4, 16, ...
+ * queries and max_batch, built from the given sample (host buffers, n queries with their k and
+ * nprobe, repeated when a batch is larger than n); afterwards every batch takes the plan tuned for
+ * the smallest size >= its query count. The server must be idle (nothing submitted and not yet
+ * completed). Not in the reference. */
+pdxg_status pdxg_server_autotune(pdxg_server* server, const float* queries, const uint32_t* k, const uint32_t* nprobe,
+                                 uint32_t n);
+/* The tuned plans, one row of 5 values each: [largest batch, meso block, data block, two-phase
+ * meso scan 0/1, scan order mode 0/1/2 (pdxg_workspace_set_order)]; *num_rows = the number of rows
+ * (0 before pdxg_server_autotune). */
+pdxg_status pdxg_server_get_plans(pdxg_server* server, uint32_t* out, uint32_t max_rows, uint32_t* num_rows);
 /* Load driver: submit query i (row i of queries, k[i], nprobe[i]) at arrival_s[i] seconds after the
  * call starts (ascending), collect every result, and write its submit-to-completion latency in ms.
  * Returns the wall time from first arrival to last completion in *elapsed_s. */
@@ -243,7 +264,11 @@ pdxg_status pdxg_server_replay(pdxg_server* server, const float* queries, const 
  * passing row ids of PDXIndex::FilteredSearch (index.hpp:273-280, 833-845).
  * ------------------------------------------------------------------------------------------- */
 typedef struct pdxg_dense pdxg_dense;
-enum { PDXG_DENSE_BF16 = 0, PDXG_DENSE_FP8 = 1 };
+/* Precision of the copy; PDXG_DENSE_NO_RESIDUAL (or-ed in) turns residual coding off. By default,
+ * when the level has centroids, the low-precision rows hold x - c (c: the centroid of the row's
+ * cluster; appended rows take their nearest centroid) and q.x = q.c + q.(x - c), with q.c from an
+ * FP32 GEMM per search. */
+enum { PDXG_DENSE_BF16 = 0, PDXG_DENSE_FP8 = 1, PDXG_DENSE_NO_RESIDUAL = 0x100 };
 
 /* Device buffers; the call is asynchronous on `stream` except when a selection is given (the
  * passing-row count is read back to size the GEMM). */
@@ -261,7 +286,7 @@ typedef struct {
 } pdxg_dense_search_args;
 
 /* Builds the copy from the index's data level (PDXG_F32 / PDXG_TREE_F32). extra_rows reserves
- * room for appended rows, whose ids may go up to (largest id + extra_rows). */
+ * room for appended rows, whose ids may go up to (largest id + extra_rows). precision: see above. */
 pdxg_status pdxg_dense_create(pdxg_index* index, uint32_t precision, uint64_t extra_rows, uint32_t max_queries,
                               pdxg_dense** out);
 pdxg_status pdxg_dense_destroy(pdxg_dense* dense);
@@ -279,8 +304,25 @@ pdxg_status pdxg_dense_delete(pdxg_dense* dense, const uint32_t* ids, uint32_t n
 pdxg_status pdxg_dense_enable_timing(pdxg_dense* dense, int enable);
 pdxg_status pdxg_dense_stage_times(pdxg_dense* dense, float* out4);
 /* [0] rows (including deleted ones), [1] row capacity, [2] id capacity, [3] device bytes,
- * [4] rows that entered the last search's GEMM, [5] precision. */
+ * [4] rows that entered the last search's GEMM, [5] precision | 0x100 when residual coding is on. */
 pdxg_status pdxg_dense_info(const pdxg_dense* dense, uint64_t* out6);
+
+/* ---------------------------------------------------------------------------------------------
+ * Captured searches (not in the reference): one pdxg_search_device call recorded into a HIP graph
+ * and replayed with a single launch, for callers that repeat a search of fixed shape (a serving
+ * loop's slot, a fixed batch size, single queries). The graph keeps the call's device pointers,
+ * counts and the workspace's plan as they were at capture; each replay searches what those
+ * buffers hold when it runs. Re-create the graph after pdxg_index_update_cluster or a plan change.
+ * While the graph exists the workspace belongs to it: replays must not overlap each other or
+ * other searches on that workspace.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct pdxg_graph pdxg_graph;
+pdxg_status pdxg_graph_create(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,
+                              pdxg_graph** out);
+pdxg_status pdxg_graph_launch(pdxg_graph* graph, void* stream);
+/* Nodes (kernel launches) of the captured search. */
+uint32_t pdxg_graph_nodes(const pdxg_graph* graph);
+pdxg_status pdxg_graph_destroy(pdxg_graph* graph);
 
 const char* pdxg_last_error(void);
 

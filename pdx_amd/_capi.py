@@ -78,7 +78,7 @@ class DenseSearchArgs(ctypes.Structure):
                 ("out_distances", ctypes.c_void_p), ("out_counts", ctypes.c_void_p)]
 
 
-PDXG_DENSE_BF16, PDXG_DENSE_FP8 = 0, 1
+PDXG_DENSE_BF16, PDXG_DENSE_FP8, PDXG_DENSE_NO_RESIDUAL = 0, 1, 0x100
 
 # every symbol include/pdx_amd.h declares
 EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_index_device_bytes",
@@ -87,10 +87,13 @@ EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster
            "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times",
            "pdxg_workspace_stage_bytes", "pdxg_workspace_meso_times", "pdxg_workspace_meso_pairs",
            "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune",
-           "pdxg_workspace_set_meso_summary", "pdxg_workspace_meso_summary", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
-           "pdxg_server_get_stats", "pdxg_server_replay", "pdxg_dense_create", "pdxg_dense_destroy",
+           "pdxg_workspace_set_meso_summary", "pdxg_workspace_meso_summary", "pdxg_workspace_set_order",
+           "pdxg_workspace_order", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit", "pdxg_server_poll",
+           "pdxg_server_get_stats", "pdxg_server_replay", "pdxg_server_autotune", "pdxg_server_get_plans",
+           "pdxg_dense_create", "pdxg_dense_destroy",
            "pdxg_dense_search", "pdxg_dense_append", "pdxg_dense_delete", "pdxg_dense_info",
-           "pdxg_dense_enable_timing", "pdxg_dense_stage_times"]
+           "pdxg_dense_enable_timing", "pdxg_dense_stage_times", "pdxg_graph_create", "pdxg_graph_launch",
+           "pdxg_graph_nodes", "pdxg_graph_destroy"]
 
 _lib = None
 
@@ -134,6 +137,10 @@ def lib():
     L.pdxg_workspace_set_meso_summary.restype = ctypes.c_int
     L.pdxg_workspace_meso_summary.argtypes = [vp]
     L.pdxg_workspace_meso_summary.restype = ctypes.c_int
+    L.pdxg_workspace_set_order.argtypes = [vp, ctypes.c_int]
+    L.pdxg_workspace_set_order.restype = ctypes.c_int
+    L.pdxg_workspace_order.argtypes = [vp]
+    L.pdxg_workspace_order.restype = ctypes.c_int
     u64p, dblp = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)
     L.pdxg_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     L.pdxg_server_destroy.argtypes = [vp]
@@ -141,6 +148,8 @@ def lib():
     L.pdxg_server_poll.argtypes = [vp, u32, ctypes.c_int, u64p, u32, u32p, f32p, u32p, f32p, u32p]
     L.pdxg_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     L.pdxg_server_replay.argtypes = [vp, f32p, u32p, u32p, dblp, u32, f32p, dblp]
+    L.pdxg_server_autotune.argtypes = [vp, f32p, u32p, u32p, u32]
+    L.pdxg_server_get_plans.argtypes = [vp, u32p, u32, u32p]
     L.pdxg_dense_create.argtypes = [vp, u32, ctypes.c_uint64, u32, ctypes.POINTER(vp)]
     L.pdxg_dense_destroy.argtypes = [vp]
     L.pdxg_dense_search.argtypes = [vp, ctypes.POINTER(DenseSearchArgs), vp]
@@ -149,6 +158,14 @@ def lib():
     L.pdxg_dense_info.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.pdxg_dense_enable_timing.argtypes = [vp, ctypes.c_int]
     L.pdxg_dense_stage_times.argtypes = [vp, f32p]
+    L.pdxg_graph_create.argtypes = [vp, vp, ctypes.POINTER(SearchArgs), ctypes.POINTER(vp)]
+    L.pdxg_graph_create.restype = ctypes.c_int
+    L.pdxg_graph_launch.argtypes = [vp, vp]
+    L.pdxg_graph_launch.restype = ctypes.c_int
+    L.pdxg_graph_nodes.argtypes = [vp]
+    L.pdxg_graph_nodes.restype = ctypes.c_uint32
+    L.pdxg_graph_destroy.argtypes = [vp]
+    L.pdxg_graph_destroy.restype = ctypes.c_int
     for name in ("pdxg_dense_create", "pdxg_dense_destroy", "pdxg_dense_search", "pdxg_dense_append",
                  "pdxg_dense_delete", "pdxg_dense_info", "pdxg_dense_enable_timing", "pdxg_dense_stage_times"):
         getattr(L, name).restype = ctypes.c_int
@@ -157,7 +174,8 @@ def lib():
                  "pdxg_filtered_search", "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing",
                  "pdxg_workspace_stage_times", "pdxg_workspace_stage_bytes", "pdxg_workspace_set_plan",
                  "pdxg_workspace_get_plan", "pdxg_workspace_autotune", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit",
-                 "pdxg_server_poll", "pdxg_server_get_stats", "pdxg_server_replay"):
+                 "pdxg_server_poll", "pdxg_server_get_stats", "pdxg_server_replay", "pdxg_server_autotune",
+                 "pdxg_server_get_plans"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -327,6 +345,14 @@ class Workspace:
     def set_meso_summary(self, enable: bool) -> None:
         check(lib().pdxg_workspace_set_meso_summary(self._h, int(enable)))
 
+    def set_order(self, mode: int) -> None:
+        """0 query order, 1 longest first, 2 longest first through a persistent work queue."""
+        check(lib().pdxg_workspace_set_order(self._h, int(mode)))
+
+    @property
+    def order(self) -> int:
+        return int(lib().pdxg_workspace_order(self._h))
+
     @property
     def meso_summary(self) -> bool:
         return bool(lib().pdxg_workspace_meso_summary(self._h))
@@ -373,6 +399,43 @@ class Workspace:
                        nprobe_all, k_max, nprobe_max, out_stride, out_ids or None, out_dists or None,
                        out_idists or None, out_counts or None, out_bytes or None)
         check(lib().pdxg_search_device(self.index.handle, self._h, ctypes.byref(a), ctypes.c_void_p(stream or 0)))
+
+    def capture(self, queries_ptr: int, nq: int, k_ptr: int, nprobe_ptr: int, k_max: int, nprobe_max: int,
+                out_stride: int, out_ids: int, out_dists: int, out_idists: int, out_counts: int,
+                out_bytes: int = 0, transformed: bool = False, k_all: int = 0, nprobe_all: int = 0) -> "Graph":
+        """The search `search` would run with these arguments, captured into a HIP graph."""
+        a = SearchArgs(nq, queries_ptr or None, int(transformed), k_ptr or None, nprobe_ptr or None, k_all,
+                       nprobe_all, k_max, nprobe_max, out_stride, out_ids or None, out_dists or None,
+                       out_idists or None, out_counts or None, out_bytes or None)
+        h = ctypes.c_void_p()
+        check(lib().pdxg_graph_create(self.index.handle, self._h, ctypes.byref(a), ctypes.byref(h)))
+        return Graph(h, self)
+
+
+class Graph:
+    """A captured pdxg_search_device call (pdxg_graph_*); replays read the captured buffers."""
+
+    def __init__(self, handle, workspace):
+        self._h = handle
+        self.workspace = workspace  # kept alive while the graph exists
+
+    @property
+    def nodes(self) -> int:
+        return int(lib().pdxg_graph_nodes(self._h))
+
+    def launch(self, stream: int = 0) -> None:
+        check(lib().pdxg_graph_launch(self._h, ctypes.c_void_p(stream or 0)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pdxg_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Dense:
@@ -479,6 +542,25 @@ class Server:
         s = ServerStats()
         check(lib().pdxg_server_get_stats(self._h, ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def autotune(self, queries: np.ndarray, k, nprobe):
+        """Tune the scan plan per batch size on a query sample (idle server); returns the plans as
+        rows of (largest batch, meso block, data block, two-phase meso scan, scan order mode)."""
+        q = np.ascontiguousarray(queries, np.float32)
+        n = q.shape[0]
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (n,)))
+        npb = np.ascontiguousarray(np.broadcast_to(np.asarray(nprobe, np.uint32), (n,)))
+        check(lib().pdxg_server_autotune(self._h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         kk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                         npb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n))
+        return self.plans()
+
+    def plans(self):
+        out = np.zeros((32, 5), np.uint32)
+        n = ctypes.c_uint32()
+        check(lib().pdxg_server_get_plans(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 32,
+                                          ctypes.byref(n)))
+        return [tuple(int(v) for v in out[r]) for r in range(min(n.value, 32))]
 
     def replay(self, queries: np.ndarray, k, nprobe, arrival_s: np.ndarray):
         """Timed load: query i is submitted at arrival_s[i]; returns (latency_ms per query, elapsed_s)."""

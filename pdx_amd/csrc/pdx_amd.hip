@@ -288,6 +288,10 @@ struct pdxg_workspace {
     uint32_t max_queries = 0, max_k = 0, max_nprobe = 0;
     uint32_t k0_max = 0, order_stride = 0;
     uint32_t block[2] = {0, 0};  // scan workgroup size per level (meso, data); 0 = heuristic
+    // order of the per-query scans: 0 query order, 1 longest walk first (k_lpt_order), 2 longest
+    // first through a persistent grid's work queue
+    uint32_t order_mode = 1;
+    DevBuf<uint32_t> perm, queue;
     DevBuf<float> qnorm, qrot, probe;
     DevBuf<uint8_t> qu8;
     DevBuf<uint32_t> k1, nv1, k0, nv0, order, ids0, cnt0;
@@ -300,7 +304,7 @@ struct pdxg_workspace {
     DevBuf<uint32_t> m_cnt, m_cur, m_lstart, m_cstart, m_list, m_soff;
     DevBuf<float> summ;
     size_t bytes() const {
-        return qnorm.bytes() + qrot.bytes() + probe.bytes() + qu8.bytes() + k1.bytes() + nv1.bytes() + k0.bytes() +
+        return perm.bytes() + queue.bytes() + qnorm.bytes() + qrot.bytes() + probe.bytes() + qu8.bytes() + k1.bytes() + nv1.bytes() + k0.bytes() +
                nv0.bytes() + order.bytes() + ids0.bytes() + cnt0.bytes() + m_cnt.bytes() + m_cur.bytes() +
                m_lstart.bytes() + m_cstart.bytes() + m_list.bytes() + m_soff.bytes() + summ.bytes();
     }
@@ -314,6 +318,7 @@ struct pdxg_index {
     Level l1, l0;
     DevBuf<float> rotation, ratios;
     int max_lds = 65536;
+    int num_cus = 1;
     std::mutex mu;  // serialises the host-buffer API (shared workspace)
     std::unique_ptr<pdxg_workspace> ws;
     DevBuf<float> h_queries, h_dists;
@@ -391,7 +396,7 @@ size_t workspace_bytes_for(const pdxg_index* ix, uint32_t nq, uint32_t max_k, ui
     b += 2 * (size_t)nq * d * 4;          // qnorm, qrot
     b += (size_t)nq * nc_probe * 4;        // probe
     b += ix->u8() ? (size_t)nq * d : 0;    // qu8
-    b += 4 * (size_t)nq * 4;               // k1 nv1 k0 nv0
+    b += 5 * (size_t)nq * 4;               // k1 nv1 k0 nv0 perm
     b += (size_t)nq * ws.order_stride * 4; // order
     b += (size_t)nq * ws.k0_max * 4 + (size_t)nq * 4;
     if (ws.summ_ok) {
@@ -414,6 +419,8 @@ void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
     ws.nv1.alloc(nq);
     ws.k0.alloc(nq);
     ws.nv0.alloc(nq);
+    ws.perm.alloc(nq);
+    ws.queue.alloc(2);
     ws.order.alloc(nq * ws.order_stride);
     if (ix->tree()) {
         ws.ids0.alloc(nq * ws.k0_max);
@@ -458,7 +465,16 @@ void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_
         HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)p.bytes));
     if (a.nq == 0) return;
-    hipLaunchKernelGGL(kern, dim3(a.nq), dim3(BLOCK), p.bytes, st, a);
+    uint32_t grid = a.nq;
+    if (a.queue) {  // persistent: as many workgroups as are resident at once
+        require(p.bytes + 64 <= (uint32_t)ix->max_lds, "no LDS left for the work-queue slot", PDXG_ERR_UNSUPPORTED);
+        int per_cu = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), BLOCK,
+                                                               p.bytes));
+        require(per_cu >= 1, "scan workgroup does not fit a CU", PDXG_ERR_UNSUPPORTED);
+        grid = std::min<uint32_t>(a.nq, (uint32_t)per_cu * (uint32_t)ix->num_cus);
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), p.bytes, st, a);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -629,6 +645,18 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
     const float* qrot = ws.qrot.p;
 
     ScanArgs base{};
+    if (ws.order_mode >= 1 && nq > 1) {  // the data-level visit count orders both levels' scans
+        hipLaunchKernelGGL(k_lpt_order, dim3(1), dim3(kLptThreads), 0, st, ws.nv1.p, nq, ws.perm.p);
+        HIP_CHECK(hipGetLastError());
+        base.perm = ws.perm.p;
+    }
+    uint32_t* queues[2] = {nullptr, nullptr};  // work-queue counters of the meso and data scans
+    if (ws.order_mode == 2) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ws.queue.p, 2u, 0u);
+        HIP_CHECK(hipGetLastError());
+        queues[0] = ws.queue.p;
+        queues[1] = ws.queue.p + 1;
+    }
     base.ratios = ix->ratios.p;
     base.out_bytes = reinterpret_cast<unsigned long long*>(a.out_bytes);
     base.nq = nq;
@@ -653,6 +681,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s.out_counts = a.out_counts;
         s.out_stride = a.out_stride;
         s.stage_bytes = sb ? sb + 1 : nullptr;
+        s.queue = queues[1];
         if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s, a.k_max, st, ws.block[1]);
         else launch_scan<false, false>(ix, ix->l1, s, a.k_max, st, ws.block[1]);
         ws.mark(4, st);
@@ -675,6 +704,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s0.out_counts = ws.cnt0.p;
         s0.out_stride = ws.k0_max;
         s0.stage_bytes = sb;
+        s0.queue = queues[0];
         const bool summ = ws.summ_ok && ws.summ_use && ix->l0.tomb_total == 0 &&
                           ix->l0.top_prefix[ws.nv0_max] <= ws.region;
         ws.last_summ = summ;
@@ -706,6 +736,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s1.out_counts = a.out_counts;
         s1.out_stride = a.out_stride;
         s1.stage_bytes = sb ? sb + 1 : nullptr;
+        s1.queue = queues[1];
         if (ix->u8()) launch_scan<true, false>(ix, ix->l1, s1, a.k_max, st, ws.block[1]);
         else launch_scan<false, false>(ix, ix->l1, s1, a.k_max, st, ws.block[1]);
         ws.mark(4, st);
@@ -730,6 +761,82 @@ pdxg_workspace* host_workspace(pdxg_index* ix, uint32_t nq, uint32_t max_k, uint
         alloc_workspace(ix, *ws);
     }
     return ws.get();
+}
+
+// Runs the given search with every scan workgroup size, level by level, and keeps the fastest
+// (for the meso level also the two-phase scan against the pruned one).
+void autotune_workspace(pdxg_index* ix, pdxg_workspace* ws, const pdxg_search_args* args, hipStream_t st) {
+    {
+        const bool was_timing = ws->timing;
+        if (!was_timing) {
+            for (auto& e : ws->ev)
+                if (!e) HIP_CHECK(hipEventCreate(&e));
+            if (!ws->ev_summ) HIP_CHECK(hipEventCreate(&ws->ev_summ));
+            ws->timing = true;
+        }
+        // tune the meso level first (two-level indexes), then the data level; each candidate runs
+        // twice and the second run's stage time counts (results do not depend on the block size)
+        // (the meso level also tries the two-phase scan against the pruned one)
+        for (int lvl = ix->tree() ? 0 : 1; lvl <= 1; ++lvl) {
+            float best = 1e30f;
+            uint32_t best_b = 0;
+            bool best_summ = false;
+            const bool summ_before = ws->summ_use;
+            for (int summ = (lvl == 0 && ws->summ_ok) ? 1 : 0; summ >= 0; --summ) {
+                if (lvl == 0) ws->summ_use = summ != 0;
+                for (uint32_t b : kBlocks) {
+                    ws->block[lvl] = b;
+                    float ms[4] = {0, 0, 0, 0};
+                    bool ok = true;
+                    for (int rep = 0; rep < 2 && ok; ++rep) {
+                        try {
+                            search_device(ix, *ws, *args, st);
+                            HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+                            for (int i = 0; i < 4; ++i)
+                                HIP_CHECK(hipEventElapsedTime(&ms[i], ws->ev[i], ws->ev[i + 1]));
+                        } catch (const Error& e) {
+                            if (e.code != PDXG_ERR_UNSUPPORTED) throw;
+                            ok = false;  // e.g. the LDS plan of this block size does not fit
+                        }
+                    }
+                    const float t = lvl == 0 ? ms[2] : ms[3];
+                    if (ok && t < best) {
+                        best = t;
+                        best_b = b;
+                        best_summ = summ != 0;
+                    }
+                }
+            }
+            require(best_b != 0, "no scan block size fits this index", PDXG_ERR_UNSUPPORTED);
+            ws->block[lvl] = best_b;
+            ws->summ_use = lvl == 0 ? best_summ : summ_before;
+        }
+        // order of the per-query scans (both levels): query order, longest walk first, longest
+        // first through a persistent work queue
+        float best_t = 1e30f;
+        uint32_t best_mode = 1;
+        for (uint32_t mode = 0; mode < 3; ++mode) {
+            ws->order_mode = mode;
+            float ms = 0.0f;
+            bool ok = true;
+            for (int rep = 0; rep < 2 && ok; ++rep) {
+                try {
+                    search_device(ix, *ws, *args, st);
+                    HIP_CHECK(hipEventSynchronize(ws->ev[4]));
+                    HIP_CHECK(hipEventElapsedTime(&ms, ws->ev[2], ws->ev[4]));
+                } catch (const Error& e) {
+                    if (e.code != PDXG_ERR_UNSUPPORTED) throw;
+                    ok = false;
+                }
+            }
+            if (ok && ms < best_t) {
+                best_t = ms;
+                best_mode = mode;
+            }
+        }
+        ws->order_mode = best_mode;
+        ws->timing = was_timing;
+    }
 }
 
 }  // namespace
@@ -773,6 +880,7 @@ pdxg_status pdxg_index_create(int device, const pdxg_index_desc* desc, pdxg_inde
             ix->inv_qscale_sq = 1.0f / ix->qscale_sq;
         }
         HIP_CHECK(hipDeviceGetAttribute(&ix->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        HIP_CHECK(hipDeviceGetAttribute(&ix->num_cus, hipDeviceAttributeMultiprocessorCount, device));
         ix->l1.create(desc->l1, ix->d, ix->v, ix->h, ix->u8());
         if (ix->tree()) {
             // the L0 search returns these ids as the L1 visit order: they must name L1 clusters
@@ -933,6 +1041,16 @@ pdxg_status pdxg_workspace_set_meso_summary(pdxg_workspace* ws, int enable) {
     });
 }
 
+pdxg_status pdxg_workspace_set_order(pdxg_workspace* ws, int mode) {
+    return guarded([&] {
+        require(ws, "null argument");
+        require(mode >= 0 && mode <= 2, "order mode must be 0, 1 or 2");
+        ws->order_mode = static_cast<uint32_t>(mode);
+    });
+}
+
+int pdxg_workspace_order(const pdxg_workspace* ws) { return ws ? static_cast<int>(ws->order_mode) : -1; }
+
 int pdxg_workspace_meso_summary(const pdxg_workspace* ws) { return ws && ws->summ_ok && ws->summ_use ? 1 : 0; }
 
 pdxg_status pdxg_workspace_get_plan(pdxg_workspace* ws, uint32_t* out2) {
@@ -947,52 +1065,7 @@ pdxg_status pdxg_workspace_autotune(pdxg_index* ix, pdxg_workspace* ws, const pd
     return guarded([&] {
         require(ix && ws && args, "null argument");
         DeviceGuard g(ix->device);
-        hipStream_t st = static_cast<hipStream_t>(stream);
-        const bool was_timing = ws->timing;
-        if (!was_timing) {
-            for (auto& e : ws->ev)
-                if (!e) HIP_CHECK(hipEventCreate(&e));
-            if (!ws->ev_summ) HIP_CHECK(hipEventCreate(&ws->ev_summ));
-            ws->timing = true;
-        }
-        // tune the meso level first (two-level indexes), then the data level; each candidate runs
-        // twice and the second run's stage time counts (results do not depend on the block size)
-        // (the meso level also tries the two-phase scan against the pruned one)
-        for (int lvl = ix->tree() ? 0 : 1; lvl <= 1; ++lvl) {
-            float best = 1e30f;
-            uint32_t best_b = 0;
-            bool best_summ = false;
-            const bool summ_before = ws->summ_use;
-            for (int summ = (lvl == 0 && ws->summ_ok) ? 1 : 0; summ >= 0; --summ) {
-                if (lvl == 0) ws->summ_use = summ != 0;
-                for (uint32_t b : kBlocks) {
-                    ws->block[lvl] = b;
-                    float ms[4] = {0, 0, 0, 0};
-                    bool ok = true;
-                    for (int rep = 0; rep < 2 && ok; ++rep) {
-                        try {
-                            search_device(ix, *ws, *args, st);
-                            HIP_CHECK(hipEventSynchronize(ws->ev[4]));
-                            for (int i = 0; i < 4; ++i)
-                                HIP_CHECK(hipEventElapsedTime(&ms[i], ws->ev[i], ws->ev[i + 1]));
-                        } catch (const Error& e) {
-                            if (e.code != PDXG_ERR_UNSUPPORTED) throw;
-                            ok = false;  // e.g. the LDS plan of this block size does not fit
-                        }
-                    }
-                    const float t = lvl == 0 ? ms[2] : ms[3];
-                    if (ok && t < best) {
-                        best = t;
-                        best_b = b;
-                        best_summ = summ != 0;
-                    }
-                }
-            }
-            require(best_b != 0, "no scan block size fits this index", PDXG_ERR_UNSUPPORTED);
-            ws->block[lvl] = best_b;
-            ws->summ_use = lvl == 0 ? best_summ : summ_before;
-        }
-        ws->timing = was_timing;
+        autotune_workspace(ix, ws, args, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -1188,3 +1261,4 @@ pdxg_status pdxg_filtered_search(pdxg_index* ix, const float* query, uint32_t tr
 
 #include "pdx_server.inc"
 #include "pdx_dense.inc"
+#include "pdx_graph.inc"

@@ -8,6 +8,13 @@
 // first v dims, then the 64-dim horizontal blocks in the 16-lane order -- the value the PDX scan
 // computes for a vector it does not prune (searcher.hpp:596-724 via distance_computers).
 //
+// Residual coding (when the level has centroids): the low-precision rows hold r = x - c, c the
+// centroid of the row's cluster (an appended row takes its nearest centroid), and
+// q.x = q.c + q.r with q.c from an FP32 GEMM over the centroids. r is about as small as the
+// cluster's spread, so the low-precision error of q.r is that much smaller than the error of a
+// low-precision q.x (this is what lets FP8 e4m3 rank neighbours whose distances differ by less
+// than its rounding of whole rows).
+//
 // A row filter (the bitmap form of FilteredSearch's passing row ids, index.hpp:833-845) and
 // deleted rows are applied by compacting the passing rows (in row order) into a packed
 // low-precision matrix before the GEMM, so the GEMM's work scales with the passing rows.
@@ -47,10 +54,11 @@ __device__ __forceinline__ float wave_max(float m) {
 }
 
 // One workgroup (256) per used slot (blockIdx.x = slot, blockIdx.y = cluster): the FP32 row from
-// the PDX layout (layout.hpp:21-87), its FP64 squared norm, id, liveness and the largest
-// magnitude.
-__global__ __launch_bounds__(256) void k_dense_rows(LevelDev L, const unsigned long long* row_off, float* x32,
-                                                    float* xn, uint32_t* ids, uint8_t* live, float* maxabs) {
+// the PDX layout (layout.hpp:21-87), its FP64 squared norm, id, liveness, cluster and the largest
+// magnitude of what the low-precision copy will hold (x, or x - c with centroids `cent`).
+__global__ __launch_bounds__(256) void k_dense_rows(LevelDev L, const unsigned long long* row_off, const float* cent,
+                                                    float* x32, float* xn, uint32_t* ids, uint8_t* live,
+                                                    uint32_t* rcl, float* maxabs) {
     __shared__ double red[4];
     const uint32_t c = blockIdx.y, i = blockIdx.x;
     const uint32_t n = L.n_used[c];
@@ -66,13 +74,14 @@ __global__ __launch_bounds__(256) void k_dense_rows(LevelDev L, const unsigned l
                               : data[(size_t)(v + (hj & ~63u)) * S + (size_t)i * kHDim + (hj & 63u)];
         x32[r * d + j] = x;
         acc += (double)x * (double)x;
-        mx = fmaxf(mx, fabsf(x));
+        mx = fmaxf(mx, fabsf(cent ? x - cent[(size_t)c * d + j] : x));
     }
     const double t = block_sum_f64_256(acc, red);
     mx = wave_max(mx);
     if ((threadIdx.x & 63) == 0) atomic_max_pos(maxabs, mx);
     if (threadIdx.x == 0) {
         xn[r] = static_cast<float>(t);
+        rcl[r] = c;
         const uint32_t id = L.ids[L.slot_off[c] + i];
         ids[r] = id;
         const bool tomb = L.n_tomb[c] != 0 && ((L.tomb[L.tomb_off[c] + (i >> 5)] >> (i & 31)) & 1u);
@@ -95,17 +104,58 @@ __global__ void k_dense_max_id(const uint32_t* ids, unsigned long long nr, uint3
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-// Low-precision copy of rows [r0, r0 + nr): BF16 (unscaled) or FP8 e4m3 scaled by *sx. Grid-stride
-// (a dispatch holds at most 2^32 work-items per dimension; a 2M x 3072 copy has 6.1e9 elements).
-__global__ void k_dense_convert(const float* __restrict__ x32, void* xlp, unsigned long long r0, unsigned long long nr,
-                                uint32_t d, uint32_t prec, const float* sx) {
-    const unsigned long long total = nr * d, step = (unsigned long long)gridDim.x * blockDim.x;
+// Low-precision copy of rows [r0, r0 + nr): x (or x - c of the row's cluster when cent is given)
+// as BF16 (unscaled) or FP8 e4m3 scaled by *sx. One workgroup per row, grid-stride over rows (a
+// dispatch holds at most 2^32 work-items per dimension).
+__global__ __launch_bounds__(256) void k_dense_convert(const float* __restrict__ x32, void* xlp, unsigned long long r0,
+                                                       unsigned long long nr, uint32_t d, uint32_t prec, const float* sx,
+                                                       const float* __restrict__ cent, const uint32_t* __restrict__ rcl) {
     const float scale = prec == kDenseBF16 ? 1.0f : *sx;
-    for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += step) {
-        const unsigned long long g = r0 * d + e;
-        if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(xlp)[g] = __float2bfloat16(x32[g]);
-        else reinterpret_cast<__hip_fp8_storage_t*>(xlp)[g] = __hip_cvt_float_to_fp8(x32[g] * scale, __HIP_SATFINITE, __HIP_E4M3);
+    for (unsigned long long rr = blockIdx.x; rr < nr; rr += gridDim.x) {
+        const unsigned long long r = r0 + rr;
+        const float* x = x32 + r * d;
+        const float* c = cent ? cent + (size_t)rcl[r] * d : nullptr;
+        for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+            const float y = c ? x[j] - c[j] : x[j];
+            if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(xlp)[r * d + j] = __float2bfloat16(y);
+            else reinterpret_cast<__hip_fp8_storage_t*>(xlp)[r * d + j] =
+                __hip_cvt_float_to_fp8(y * scale, __HIP_SATFINITE, __HIP_E4M3);
+        }
     }
+}
+
+// FP32 squared norm of each centroid (FP64 accumulation), one workgroup per centroid.
+__global__ __launch_bounds__(256) void k_dense_cnorm(const float* cent, uint32_t d, float* cn) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+        const double x = cent[(size_t)blockIdx.x * d + j];
+        acc += x * x;
+    }
+    const double t = block_sum_f64_256(acc, red);
+    if (threadIdx.x == 0) cn[blockIdx.x] = static_cast<float>(t);
+}
+
+// Cluster of each appended row: argmin_c ||c||^2 - 2 x.c (G holds -2 x.c, column-major: column
+// t = row t over the nc centroids); ties go to the smaller cluster. One wave per row.
+__global__ __launch_bounds__(64) void k_dense_anchor(const float* __restrict__ G, const float* __restrict__ cn,
+                                                     uint32_t nc, unsigned long long r0, uint32_t* rcl) {
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    uint64_t best = kKeyInf;
+    for (uint32_t c = lane; c < nc; c += 64) {
+        const float v = cn[c] + G[(size_t)t * nc + c];
+        // order-preserving map of a float to an unsigned key (negative values included)
+        const uint32_t b = __float_as_uint(v);
+        const uint32_t o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+        const uint64_t key = (static_cast<uint64_t>(o) << 32) | c;
+        best = key < best ? key : best;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(best, off);
+        best = o < best ? o : best;
+    }
+    if (lane == 0) rcl[r0 + t] = static_cast<uint32_t>(best & 0xFFFFFFFFu);
 }
 
 // FP8 scales from a largest magnitude: s = kFp8Range / m (values are multiplied by s before the
@@ -186,8 +236,9 @@ __global__ __launch_bounds__(1024) void k_dense_offsets(const uint32_t* counts, 
 }
 
 __global__ __launch_bounds__(256) void k_dense_compact(const uint8_t* live, const uint32_t* ids, const uint8_t* sel,
-                                                       const float* xn, unsigned long long nr, const uint32_t* offs,
-                                                       uint32_t* rowlist, float* cbias) {
+                                                       const float* xn, const uint32_t* rcl, unsigned long long nr,
+                                                       const uint32_t* offs, uint32_t* rowlist, float* cbias,
+                                                       uint32_t* ccl) {
     __shared__ uint32_t red[8];
     const unsigned long long base = (unsigned long long)blockIdx.x * kCompactRows;
     uint32_t pos = offs[blockIdx.x];
@@ -200,6 +251,7 @@ __global__ __launch_bounds__(256) void k_dense_compact(const uint8_t* live, cons
         if (p) {
             rowlist[pos + ex] = static_cast<uint32_t>(r);
             cbias[pos + ex] = xn[r];
+            ccl[pos + ex] = rcl[r];
         }
         pos += tot;
     }
@@ -301,14 +353,17 @@ struct WaveTopR {
 };
 
 // One wave per (segment, query) of one GEMM chunk: the R smallest keys of the segment
-// (distance = max(0, ||q||^2 + bias[r] + qfac[q] * S[r, q]), row = rowlist[r] or row0 + r), written sorted
-// and padded with all-ones keys to part[(q * nseg + seg) * R]. The query's running list from the
+// (distance = max(0, ||q||^2 + bias[r] + qc[q, cl[r]] + qfac[q] * S[r, q]), with qc = -2 q.c of the
+// row's cluster under residual coding, else 0; row = rowlist[r] or row0 + r), written sorted and
+// padded with all-ones keys to part[(q * nseg + seg) * R]. The query's running list from the
 // earlier chunks seeds the threshold.
 __global__ __launch_bounds__(64) void k_dense_select(const float* __restrict__ S, uint32_t m, uint32_t seg_rows,
                                                      const float* __restrict__ bias, const uint32_t* __restrict__ rowlist,
                                                      unsigned long long row0, const float* __restrict__ qn,
-                                                     const float* __restrict__ qfac, const uint64_t* running,
-                                                     const uint32_t* run_len, uint32_t R, uint64_t* part) {
+                                                     const float* __restrict__ qfac, const float* __restrict__ qc,
+                                                     const uint32_t* __restrict__ cl, uint32_t nc,
+                                                     const uint64_t* running, const uint32_t* run_len, uint32_t R,
+                                                     uint64_t* part) {
     __shared__ uint64_t lds[kSelBuf + kSelMax];
     const uint32_t seg = blockIdx.x, nseg = gridDim.x, q = blockIdx.y, lane = threadIdx.x;
     WaveTopR top;
@@ -317,6 +372,7 @@ __global__ __launch_bounds__(64) void k_dense_select(const float* __restrict__ S
     const uint32_t r_begin = seg * seg_rows, r_end = min(m, r_begin + seg_rows);
     const float qq = qn[q], f = qfac[q];
     const float* col = S + (size_t)q * m;
+    const float* qcq = qc ? qc + (size_t)q * nc : nullptr;
     for (uint32_t r0 = r_begin; r0 < r_end; r0 += 256) {
         float s[4], b[4];
         uint32_t rr[4];
@@ -326,6 +382,7 @@ __global__ __launch_bounds__(64) void k_dense_select(const float* __restrict__ S
             const bool in = rr[u] < r_end;
             s[u] = in ? col[rr[u]] : 0.0f;
             b[u] = in ? bias[rr[u]] : __builtin_inff();
+            if (qcq && in) b[u] += qcq[cl[rr[u]]];
         }
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {

@@ -61,6 +61,9 @@ struct ScanArgs {
     unsigned long long region;
     uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector (nw + h/64)
     uint32_t nq;
+    const uint32_t* perm;       // optional launch order: workgroup b runs query perm[b]
+    uint32_t* queue;            // optional work-queue counter (persistent launch): queries are
+                                // taken in perm order by atomically incrementing it
     // dynamic LDS layout (byte offsets)
     uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_rat, o_ring;
 };
@@ -135,6 +138,41 @@ __global__ void k_setup(SetupArgs a) {
     a.k0[q] = np1 > a.k0_cap ? a.k0_cap : np1;
     a.nvisit0[q] = (np0 == 0 || np0 > a.nc0) ? a.nc0 : np0;
     a.nvisit1[q] = np1 > a.nc1 ? a.nc1 : np1;
+}
+
+// Longest-first launch order of the per-query scans: queries sorted by descending visit count
+// (counting sort, counts above kLptBuckets - 1 share the top bucket; order within a bucket is
+// unspecified). Workgroups are dispatched in launch order, so the longest walks start first and
+// the short ones fill in behind them. One workgroup of kLptThreads threads.
+constexpr uint32_t kLptBuckets = 4096;
+constexpr uint32_t kLptThreads = 1024;
+__global__ __launch_bounds__(kLptThreads) void k_lpt_order(const uint32_t* __restrict__ key, uint32_t nq,
+                                                           uint32_t* __restrict__ perm) {
+    __shared__ uint32_t cur[kLptBuckets];
+    __shared__ uint32_t red[kLptThreads / kWave];
+    constexpr uint32_t per = kLptBuckets / kLptThreads;
+    for (uint32_t b = threadIdx.x; b < kLptBuckets; b += kLptThreads) cur[b] = 0;
+    __syncthreads();
+    // bucket kLptBuckets - 1 - min(key, kLptBuckets - 1): ascending bucket = descending key
+    for (uint32_t q = threadIdx.x; q < nq; q += kLptThreads)
+        atomicAdd(&cur[kLptBuckets - 1 - min(key[q], kLptBuckets - 1)], 1u);
+    __syncthreads();
+    uint32_t c[per], sum = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < per; ++u) {
+        c[u] = cur[threadIdx.x * per + u];
+        sum += c[u];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_sum<kLptThreads>(sum, red, &total);
+#pragma unroll
+    for (uint32_t u = 0; u < per; ++u) {
+        cur[threadIdx.x * per + u] = off;
+        off += c[u];
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < nq; q += kLptThreads)
+        perm[atomicAdd(&cur[kLptBuckets - 1 - min(key[q], kLptBuckets - 1)], 1u)] = q;
 }
 
 __global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
@@ -1024,6 +1062,13 @@ struct Scanner {
         }
     }
 
+    // per-query state, for a persistent workgroup that runs several queries
+    __device__ void reset() {
+        hl = 0;
+        bytes = 0;
+        bytes_local = 0;
+    }
+
     __device__ void run(uint32_t q) {
         load_query(q);
         const uint32_t k = a.kq[q];
@@ -1102,12 +1147,27 @@ struct Scanner {
     }
 };
 
+// One workgroup per query; or, with a.queue, a persistent grid whose workgroups take queries from
+// the work queue (in perm order) until it is empty: every workgroup reaches the exit once the
+// counter passes nq.
 template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false>
 __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
-    if (blockIdx.x >= a.nq) return;
     Scanner<U8, FILT, BLOCK, SUMM> s(a, lds_raw);
-    s.run(blockIdx.x);
+    if (a.queue) {
+        __shared__ uint32_t next;
+        for (;;) {
+            __syncthreads();  // the previous query's results are written
+            if (threadIdx.x == 0) next = atomicAdd(a.queue, 1u);
+            __syncthreads();
+            const uint32_t i = next;
+            if (i >= a.nq) return;
+            s.reset();
+            s.run(a.perm ? a.perm[i] : i);
+        }
+    }
+    if (blockIdx.x >= a.nq) return;
+    s.run(a.perm ? a.perm[blockIdx.x] : blockIdx.x);
 }
 
 }  // namespace pdxg

@@ -8,7 +8,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     hdr = open(os.path.join(ROOT, "include", "pdx_amd.h")).read()
-    return sorted(set(re.findall(r"(?:pdxg_status|size_t|int|const char\s*\*)\s*(pdxg_\w+)\s*\(", hdr)))
+    return sorted(set(re.findall(r"(?:pdxg_status|size_t|int|uint32_t|const char\s*\*)\s*(pdxg_\w+)\s*\(", hdr)))
 
 
 def test_header_declarations_are_listed_in_binding():

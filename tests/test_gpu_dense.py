@@ -15,7 +15,8 @@ from tests.pdx_testdata import index_file, mixture
 
 pytestmark = pytest.mark.gpu
 
-PRECISIONS = [("bf16", 0), ("fp8", 1)]
+# residual coding is on by default where the level has centroids; "_direct" codes the rows themselves
+PRECISIONS = [("bf16", 0), ("fp8", 1), ("fp8_direct", 0x101)]
 
 
 @pytest.fixture(scope="module")
@@ -150,6 +151,8 @@ def test_filter_delete_append(gpu_lib, name, prec):
     run = _Run(gpu_lib, g, prec, len(q), extra_rows=64)
     info = run.dn.info()
     assert info["rows"] == len(ids) and info["id_capacity"] == int(ids.max()) + 1 + 64
+    # 0x100 reports residual coding: on unless turned off (this index has centroids)
+    assert info["precision"] == (prec & 0xFF) | (0 if prec & 0x100 else 0x100)
     rng = np.random.default_rng(3)
     sel = np.zeros(info["id_capacity"], np.uint8)
     sel[rng.choice(ids, 200, replace=False)] = 1
@@ -252,3 +255,33 @@ def test_several_gemm_chunks_and_single_query(gpu_lib, name, prec):
         si, sd, sc = run.search(q[t:t + 1], k, 4)
         np.testing.assert_array_equal(si[0], gi[t])
         np.testing.assert_array_equal(sd[0].view(np.uint32), gd[t].view(np.uint32))
+
+
+def test_residual_coding_raises_fp8_recall(gpu_lib):
+    """Tight clusters of unit-norm rows (noise norm 0.1): neighbours' distances differ by less than
+    FP8's rounding of whole rows, but not of residuals to the cluster centroid. Residual FP8 must
+    reach high recall@10 at 4x oversampling and beat FP8 of the rows themselves."""
+    n, d, nq, k = 40_000, 512, 128, 10
+    rng = np.random.default_rng(17)
+    c = rng.standard_normal((64, d)).astype(np.float32)
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    x = c[rng.integers(0, 64, n)] + np.float32(0.1 / np.sqrt(d)) * rng.standard_normal((n, d), dtype=np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    q = c[rng.integers(0, 64, nq)] + np.float32(0.1 / np.sqrt(d)) * rng.standard_normal((nq, d), dtype=np.float32)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    idx = builder.build_index(np.ascontiguousarray(x), "pdx_f32", num_clusters=128, seed=3, kmeans_iters=4,
+                              device=0)
+    g = gpu_lib.GpuIndex(idx)
+    rows, ids = _rows(idx)
+    qrot, _ = g.preprocess(np.ascontiguousarray(q))
+    q64, r64 = qrot.astype(np.float64), rows.astype(np.float64)
+    exact = np.array([ids[np.argsort(((r64 - qq) ** 2).sum(1), kind="stable")[:k]] for qq in q64])
+    rec = {}
+    for name, prec in (("fp8", 1), ("fp8_direct", 0x101), ("bf16", 0)):
+        run = _Run(gpu_lib, g, prec, nq)
+        gi, gd, gc = run.search(q, k, 4)
+        rec[name] = _recall(gi, exact, k)
+        run.dn.close()
+    print("recall@10 at 4x oversampling:", rec, flush=True)
+    assert rec["fp8"] >= 0.9 and rec["bf16"] >= 0.97
+    assert rec["fp8"] > rec["fp8_direct"]

@@ -115,14 +115,17 @@ def test_every_scan_block_size_gives_the_oracle_results(gpu_lib, kind):
     ws = gpu_lib.Workspace(g, nq, K, npb)
     sp = torch.cuda.current_stream().cuda_stream
     for b in (64, 128, 256, 512, 1024):
-        print(f"{kind} block {b}", flush=True)
-        ws.set_plan(b, b)
-        ws.search(qd.data_ptr(), nq, 0, 0, K, npb, K, ids.data_ptr(), dd.data_ptr(), 0, cnt.data_ptr(), 0,
-                  stream=sp, k_all=K, nprobe_all=npb)
-        torch.cuda.synchronize()
-        np.testing.assert_array_equal(cnt.cpu().numpy().view(np.uint32), oc)
-        np.testing.assert_array_equal(ids.cpu().numpy().view(np.uint32), oi)
-        np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), od.view(np.uint32))
+        for mode in (0, 1, 2):
+            print(f"{kind} block {b} order mode {mode}", flush=True)
+            ws.set_plan(b, b)
+            ws.set_order(mode)
+            assert ws.order == mode
+            ws.search(qd.data_ptr(), nq, 0, 0, K, npb, K, ids.data_ptr(), dd.data_ptr(), 0, cnt.data_ptr(), 0,
+                      stream=sp, k_all=K, nprobe_all=npb)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(cnt.cpu().numpy().view(np.uint32), oc)
+            np.testing.assert_array_equal(ids.cpu().numpy().view(np.uint32), oi)
+            np.testing.assert_array_equal(dd.cpu().numpy().view(np.uint32), od.view(np.uint32))
     plan = ws.autotune(qd.data_ptr(), nq, 0, 0, K, npb, K, ids.data_ptr(), dd.data_ptr(), 0, cnt.data_ptr(),
                        stream=sp, k_all=K, nprobe_all=npb)
     assert plan[1] in (64, 128, 256, 512, 1024)
@@ -163,8 +166,10 @@ def test_two_phase_meso_scan_matches_pruned_and_oracle(gpu_lib, kind, n, d, nlis
         ws.set_meso_summary(summ)
         assert ws.meso_summary == summ
         for b in (64, 256, 1024):
-            print(f"{kind} d={d} summary={summ} block {b}", flush=True)
+            mode = {64: 2, 256: 0, 1024: 1}[b]  # per-query visit counts differ: every scan order
+            print(f"{kind} d={d} summary={summ} block {b} order mode {mode}", flush=True)
             ws.set_plan(b, 0)
+            ws.set_order(mode)
             ids = torch.zeros((nq, K), dtype=torch.int32, device=dev)
             dd = torch.zeros((nq, K), dtype=torch.float32, device=dev)
             idd = torch.zeros((nq, K), dtype=torch.int32, device=dev)

@@ -82,3 +82,24 @@ def test_scratch_budget_bounds_the_slots(setup):
     with pytest.raises(_capi.PdxgError):
         s.submit(qs[0], 10, 0, 0)       # nprobe "all" above max_nprobe
     s.close()
+
+
+def test_autotuned_plans_keep_results(setup):
+    _capi, path, g, qs, k, nprobe = setup
+    sv = _capi.Server(g, max_batch=20, max_k=100, max_nprobe=0, max_slots=2)
+    assert sv.plans() == []
+    plans = sv.autotune(qs[:12], k[:12], nprobe[:12])
+    # batches of 1, 4, 16 and max_batch; blocks are workgroup sizes; the two-phase flag is 0/1
+    assert [p[0] for p in plans] == [1, 4, 16, 20]
+    assert all(p[1] in (64, 128, 256, 512, 1024) and p[2] in (64, 128, 256, 512, 1024) and p[3] in (0, 1)
+               and p[4] in (0, 1, 2) for p in plans)
+    for i in range(len(qs)):
+        sv.submit(qs[i], int(k[i]), int(nprobe[i]), i)
+    got = _collect(sv, len(qs))
+    o = OracleIndex(path)
+    oi, od, _, oc, _ = o.search(qs, k, nprobe)
+    for i in range(len(qs)):
+        ids, d, _ = got[i]
+        np.testing.assert_array_equal(ids, oi[i, :oc[i]])
+        np.testing.assert_array_equal(d.view(np.uint32), od[i, :oc[i]].view(np.uint32))
+    sv.close()
