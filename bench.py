@@ -133,9 +133,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work per baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-slots", default="1,2,4",
+    ap.add_argument("--serve-slots", default="1,2:256,2,4",
                     help="batches in flight (server slots); a list runs the serving leg once per value")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--precisions", default="", help="dense configurations: comma-separated subset of the precisions")
     ap.add_argument("--sigma-per-coord", action="store_true", help="unit-norm mixtures: sigma per coordinate")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--parity-queries", type=int, default=256)
@@ -433,10 +434,11 @@ def main():
     serving = None
     if not args.no_serve:
         # one serving leg per slot count; the headline is the one with the lowest p99 at 50% load
-        legs = [serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, int(ns)) for ns in str(args.serve_slots).split(",")]
+        specs = str(args.serve_slots).split(",")
+        legs = [serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, sp_) for sp_ in specs]
         serving = min(legs, key=lambda l: l["load"]["50%"]["p99_ms"])
-        serving["by_slots"] = {str(l["slots"]): {"saturated_qps": l["saturated_qps"],
-                                                 "p99_ms_at_50%": l["load"]["50%"]["p99_ms"]} for l in legs}
+        serving["by_slots"] = {sp_: {"saturated_qps": l["saturated_qps"], "p99_ms_at_50%": l["load"]["50%"]["p99_ms"]}
+                               for sp_, l in zip(specs, legs)}
     cpu = None if args.no_cpu else cpu_baseline(idx, q, nprobe_q, k_q, args)
 
     line = {
@@ -555,7 +557,7 @@ def bench_dense(args, cfg, torch, _capi, builder, dev):
         return rec, {"max_rel": float(errs.max()), "mean_rel": float(errs.mean())}
 
     results = {}
-    for pname in cfg["precisions"]:
+    for pname in (args.precisions.split(",") if args.precisions else cfg["precisions"]):
         # "fp8_direct": FP8 of the rows themselves (no residual coding), for comparison
         prec = {"bf16": _capi.PDXG_DENSE_BF16, "fp8": _capi.PDXG_DENSE_FP8,
                 "fp8_direct": _capi.PDXG_DENSE_FP8 | _capi.PDXG_DENSE_NO_RESIDUAL}[pname]
@@ -736,12 +738,16 @@ def check_parity(idx, q, nprobe_q, k_q, n, run, out_ids, out_d, out_i, out_c, to
     return {"queries": n, "ids_equal": ok_i, "int32_distances_equal": ok_d, "float_distances_equal": ok_f}
 
 
-def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, slots):
-    """Continuous batching: saturated QPS, then Poisson arrivals at fractions of it (p50/p99)."""
+def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, spec):
+    """Continuous batching: saturated QPS, then Poisson arrivals at fractions of it (p50/p99).
+    spec "S" or "S:P": S slots; with P, a second batch starts beside a running one only when at
+    least P requests wait."""
     K = cfg["k"]
+    slots, _, pmin = str(spec).partition(":")
+    slots, pmin = int(slots), int(pmin or 0)
     budget = int(cfg.get("scratch_gib", 0) * 2**30)  # memory-based admission: slots fit the budget
     sv = _capi.Server(g, max_batch=cfg["batch"], max_k=K, max_nprobe=int(nprobe_q.max()), max_slots=slots,
-                      scratch_budget_bytes=budget, scan_block=plan)
+                      scratch_budget_bytes=budget, scan_block=plan, pipeline_min=pmin)
     rng = np.random.default_rng(7)
     # scan plans per batch size (the batches under load are far smaller than the saturated ones)
     ns = min(len(q), cfg["batch"])
@@ -750,19 +756,25 @@ def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, slots):
     for _ in range(2):  # second pass: warm
         lat, el = sv.replay(q[:n_sat], k_q[:n_sat], nprobe_q[:n_sat], np.zeros(n_sat))
     sat = n_sat / el
-    out = {"saturated_qps": round(sat, 1), "slots": sv.stats()["slots"], "max_batch": cfg["batch"],
-           "scratch_bytes": sv.stats()["scratch_bytes"], "scratch_budget_bytes": budget, "load": {}}
+    out = {"saturated_qps": round(sat, 1), "slots": sv.stats()["slots"], "pipeline_min": pmin,
+           "max_batch": cfg["batch"], "scratch_bytes": sv.stats()["scratch_bytes"], "scratch_budget_bytes": budget,
+           "load": {}}
     for frac in cfg.get("loads", (0.1, 0.5, 0.9)):
         rate = frac * sat
         n = int(min(60000, max(2000, rate * 0.4)))
         sel = rng.integers(0, len(q), n)
         arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        s0 = sv.stats()
         lat, el = sv.replay(q[sel], k_q[sel], nprobe_q[sel], arr)
+        s1 = sv.stats()
+        nb = max(1, s1["batches"] - s0["batches"])
+        per = {f: round((s1[f] - s0[f]) / nb, 4) for f in ("gpu_ms", "host_ms", "turnaround_ms")}
         out["load"][f"{int(frac * 100)}%"] = {"offered_qps": round(rate, 1), "served_qps": round(n / el, 1),
                                               "p50_ms": round(float(np.percentile(lat, 50)), 4),
-                                              "p99_ms": round(float(np.percentile(lat, 99)), 4)}
-        log(f"serving ({slots} slots) at {int(frac * 100)}% of {sat:.0f} QPS: p50 {np.percentile(lat, 50):.3f} ms "
-            f"p99 {np.percentile(lat, 99):.3f} ms")
+                                              "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                                              "mean_batch": round(n / nb, 1), "per_batch_ms": per}
+        log(f"serving ({spec} slots) at {int(frac * 100)}% of {sat:.0f} QPS: p50 {np.percentile(lat, 50):.3f} ms "
+            f"p99 {np.percentile(lat, 99):.3f} ms; {n / nb:.0f} queries/batch, per batch ms {per}")
     st = sv.stats()
     out["mean_batch"] = round(st["completed"] / max(1, st["batches"]), 1)
     out["plans"] = [list(p) for p in ladder]

@@ -214,12 +214,17 @@ typedef struct {
     uint32_t max_slots;          /* upper bound on batches in flight (streams) */
     size_t scratch_budget_bytes; /* device scratch for all slots; 0 = no limit beyond max_slots */
     uint32_t scan_block[2];      /* scan workgroup sizes (meso, data), as pdxg_workspace_set_plan */
+    uint32_t pipeline_min;       /* a batch starts while another is in flight only when at least this
+                                    many requests wait (0 = whenever a slot is free) */
 } pdxg_server_config;
 
 typedef struct {
     uint64_t submitted, completed, batches;
     uint32_t slots;              /* slots admitted by the scratch budget */
     size_t scratch_bytes;        /* device scratch held by the slots */
+    /* sums over completed batches (ms): device time of the search (HIP events around it), host
+     * time from taking the requests to the last launch, and taking the requests to results ready */
+    double gpu_ms, host_ms, turnaround_ms;
 } pdxg_server_stats;
 
 pdxg_status pdxg_server_create(pdxg_index* index, const pdxg_server_config* config, pdxg_server** out);

@@ -54,12 +54,13 @@ class IndexDesc(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("max_k", ctypes.c_uint32), ("max_nprobe", ctypes.c_uint32),
                 ("max_slots", ctypes.c_uint32), ("scratch_budget_bytes", ctypes.c_size_t),
-                ("scan_block", ctypes.c_uint32 * 2)]
+                ("scan_block", ctypes.c_uint32 * 2), ("pipeline_min", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
-                ("slots", ctypes.c_uint32), ("scratch_bytes", ctypes.c_size_t)]
+                ("slots", ctypes.c_uint32), ("scratch_bytes", ctypes.c_size_t), ("gpu_ms", ctypes.c_double),
+                ("host_ms", ctypes.c_double), ("turnaround_ms", ctypes.c_double)]
 
 
 class SearchArgs(ctypes.Structure):
@@ -499,11 +500,11 @@ class Server:
     """Continuous-batching server over a resident index (pdxg_server_*)."""
 
     def __init__(self, index: GpuIndex, max_batch: int = 1024, max_k: int = 100, max_nprobe: int = 0,
-                 max_slots: int = 4, scratch_budget_bytes: int = 0, scan_block=(0, 0)):
+                 max_slots: int = 4, scratch_budget_bytes: int = 0, scan_block=(0, 0), pipeline_min: int = 0):
         self.index = index
         self.max_k = max_k
         cfg = ServerConfig(max_batch, max_k, max_nprobe, max_slots, scratch_budget_bytes,
-                           (ctypes.c_uint32 * 2)(*scan_block))
+                           (ctypes.c_uint32 * 2)(*scan_block), pipeline_min)
         h = ctypes.c_void_p()
         check(lib().pdxg_server_create(index.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

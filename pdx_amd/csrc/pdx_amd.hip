@@ -460,21 +460,31 @@ void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_
     a.o_tau = p.o_tau;
     a.o_rat = p.o_rat;
     a.o_ring = p.o_ring;
-    auto kern = k_scan<U8, FILT, BLOCK, SUMM>;
+    if constexpr (!FILT) {
+        if (a.queue) {  // persistent: as many workgroups as are resident at once
+            require(p.bytes + 64 <= (uint32_t)ix->max_lds, "no LDS left for the work-queue slot", PDXG_ERR_UNSUPPORTED);
+            auto kern = k_scan<U8, FILT, BLOCK, SUMM, true>;
+            if (p.bytes > 65536)
+                HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.bytes));
+            if (a.nq == 0) return;
+            int per_cu = 0;
+            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), BLOCK,
+                                                                   p.bytes));
+            require(per_cu >= 1, "scan workgroup does not fit a CU", PDXG_ERR_UNSUPPORTED);
+            const uint32_t grid = std::min<uint32_t>(a.nq, (uint32_t)per_cu * (uint32_t)ix->num_cus);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), p.bytes, st, a);
+            HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
+    require(!a.queue, "no persistent filtered scan", PDXG_ERR_UNSUPPORTED);
+    auto kern = k_scan<U8, FILT, BLOCK, SUMM, false>;
     if (p.bytes > 65536)
         HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)p.bytes));
     if (a.nq == 0) return;
-    uint32_t grid = a.nq;
-    if (a.queue) {  // persistent: as many workgroups as are resident at once
-        require(p.bytes + 64 <= (uint32_t)ix->max_lds, "no LDS left for the work-queue slot", PDXG_ERR_UNSUPPORTED);
-        int per_cu = 0;
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), BLOCK,
-                                                               p.bytes));
-        require(per_cu >= 1, "scan workgroup does not fit a CU", PDXG_ERR_UNSUPPORTED);
-        grid = std::min<uint32_t>(a.nq, (uint32_t)per_cu * (uint32_t)ix->num_cus);
-    }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), p.bytes, st, a);
+    hipLaunchKernelGGL(kern, dim3(a.nq), dim3(BLOCK), p.bytes, st, a);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -1062,13 +1062,6 @@ struct Scanner {
         }
     }
 
-    // per-query state, for a persistent workgroup that runs several queries
-    __device__ void reset() {
-        hl = 0;
-        bytes = 0;
-        bytes_local = 0;
-    }
-
     __device__ void run(uint32_t q) {
         load_query(q);
         const uint32_t k = a.kq[q];
@@ -1147,14 +1140,14 @@ struct Scanner {
     }
 };
 
-// One workgroup per query; or, with a.queue, a persistent grid whose workgroups take queries from
-// the work queue (in perm order) until it is empty: every workgroup reaches the exit once the
-// counter passes nq.
-template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false>
+// One workgroup per query (PERSIST = false); or a persistent grid whose workgroups take queries
+// from the work queue a.queue (in perm order) until it is empty: every workgroup reaches the exit
+// once the counter passes nq. A fresh Scanner per query keeps no state across the loop (the
+// persistent instantiation is separate so the one-query kernel's code is unchanged by it).
+template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false, bool PERSIST = false>
 __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
-    Scanner<U8, FILT, BLOCK, SUMM> s(a, lds_raw);
-    if (a.queue) {
+    if constexpr (PERSIST) {
         __shared__ uint32_t next;
         for (;;) {
             __syncthreads();  // the previous query's results are written
@@ -1162,12 +1155,14 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a) {
             __syncthreads();
             const uint32_t i = next;
             if (i >= a.nq) return;
-            s.reset();
+            Scanner<U8, FILT, BLOCK, SUMM> s(a, lds_raw);
             s.run(a.perm ? a.perm[i] : i);
         }
+    } else {
+        if (blockIdx.x >= a.nq) return;
+        Scanner<U8, FILT, BLOCK, SUMM> s(a, lds_raw);
+        s.run(a.perm ? a.perm[blockIdx.x] : blockIdx.x);
     }
-    if (blockIdx.x >= a.nq) return;
-    s.run(a.perm ? a.perm[blockIdx.x] : blockIdx.x);
 }
 
 }  // namespace pdxg
