@@ -293,3 +293,19 @@ def test_deleted_slots_are_never_returned_when_enough_live_vectors():
             break
     after = o.search(qr, 10, 4, transformed=True)[0]
     assert victim not in after[0].tolist()
+
+
+def test_rotation_is_the_matrix_product_and_preserves_norms():
+    """PreprocessQuery: y = x . M with M the stored row-major matrix (adsampling.hpp:225-245). The
+    oracle's fma chain agrees with an FP64 matrix product to FP32 rounding, and the orthogonal
+    matrix keeps L2 norms (the reference's cosine = L2 test relies on it, test_search.cpp:289-333)."""
+    path, x, q = index_file("pdx_f32", 2000, 96, 8)
+    idx = pf.load(path)
+    o = OracleIndex(path)
+    y = o.preprocess(q)
+    ref = q.astype(np.float64) @ idx.matrix.astype(np.float64)
+    np.testing.assert_allclose(y, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+    np.testing.assert_allclose(np.linalg.norm(y, axis=1), np.linalg.norm(q.astype(np.float64), axis=1), rtol=1e-5)
+    # a transposed matrix is a different map: the test would catch y = M . x
+    wrong = q.astype(np.float64) @ idx.matrix.astype(np.float64).T
+    assert np.abs(wrong - ref).max() > 1e-2
