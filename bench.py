@@ -81,6 +81,8 @@ def mixture(n, d, n_centers, sigma, seed, n_queries, unit=True, per_coord=False)
         if unit:
             blk /= np.linalg.norm(blk, axis=1, keepdims=True)
         x[s:s + m] = blk
+        if (s // chunk) % 16 == 15:
+            log(f"data: {s + m} of {n} rows")
     q = c[rng.integers(0, n_centers, n_queries)] + np.float32(sigma) * rng.standard_normal((n_queries, d), dtype=np.float32)
     if unit:
         q /= np.linalg.norm(q, axis=1, keepdims=True)

@@ -373,22 +373,30 @@ __global__ __launch_bounds__(64) void k_dense_select(const float* __restrict__ S
     const float qq = qn[q], f = qfac[q];
     const float* col = S + (size_t)q * m;
     const float* qcq = qc ? qc + (size_t)q * nc : nullptr;
-    for (uint32_t r0 = r_begin; r0 < r_end; r0 += 256) {
-        float s[4], b[4];
-        uint32_t rr[4];
+    constexpr uint32_t U = 8;  // rows per lane per step, all loads issued before the offers
+    for (uint32_t r0 = r_begin; r0 < r_end; r0 += U * 64) {
+        float s[U], b[U];
+        uint32_t rr[U], row[U];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
+        for (uint32_t u = 0; u < U; ++u) {
             rr[u] = r0 + u * 64 + lane;
             const bool in = rr[u] < r_end;
             s[u] = in ? col[rr[u]] : 0.0f;
             b[u] = in ? bias[rr[u]] : __builtin_inff();
-            if (qcq && in) b[u] += qcq[cl[rr[u]]];
+            row[u] = rowlist ? (in ? rowlist[rr[u]] : 0u) : static_cast<uint32_t>(row0 + rr[u]);
+        }
+        if (qcq) {
+            uint32_t c[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) c[u] = rr[u] < r_end ? cl[rr[u]] : 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                if (rr[u] < r_end) b[u] += qcq[c[u]];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
+        for (uint32_t u = 0; u < U; ++u) {
             const float dist = fmaxf(qq + b[u] + f * s[u], 0.0f);
-            const uint32_t row = rowlist ? (rr[u] < r_end ? rowlist[rr[u]] : 0u) : static_cast<uint32_t>(row0 + rr[u]);
-            top.offer(make_key(dist, row), dist < __builtin_inff());
+            top.offer(make_key(dist, row[u]), dist < __builtin_inff());
         }
     }
     top.finish();
