@@ -598,10 +598,16 @@ void launch_preprocess(const pdxg_index* ix, pdxg_workspace& ws, const float* qu
         HIP_CHECK(hipGetLastError());
         src = ws.qnorm.p;
     }
-    dim3 g((ix->d + kRotTI - 1) / kRotTI, (nq + kRotTQ - 1) / kRotTQ);
-    hipLaunchKernelGGL(k_rotate, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p,
-                       ix->u8() ? ws.qu8.p : static_cast<uint8_t*>(nullptr), ix->d, nq, ix->qbase, ix->qscale,
-                       transformed ? 1 : 0);
+    uint8_t* yq = ix->u8() ? ws.qu8.p : static_cast<uint8_t*>(nullptr);
+    if (!transformed && nq >= kRotBQ) {  // batches: register-blocked tiles
+        dim3 g((ix->d + kRotBI - 1) / kRotBI, (nq + kRotBQ - 1) / kRotBQ);
+        hipLaunchKernelGGL(k_rotate_tiled, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p, yq, ix->d, nq,
+                           ix->qbase, ix->qscale);
+    } else {
+        dim3 g((ix->d + kRotTI - 1) / kRotTI, (nq + kRotTQ - 1) / kRotTQ);
+        hipLaunchKernelGGL(k_rotate, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p, yq, ix->d, nq, ix->qbase,
+                           ix->qscale, transformed ? 1 : 0);
+    }
     HIP_CHECK(hipGetLastError());
 }
 

@@ -260,6 +260,68 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ x, con
 // dist[q][c] = l2sq16(q, centroid c). One workgroup per (QT queries, 64 centroids): the QT queries
 // sit in LDS and each 16-lane row streams one centroid at a time, reusing every centroid load for
 // all QT queries (QT accumulators per lane, each in the 16-lane order of l2sq16).
+// Batched rotation (nq >= kRotBQ): 64 queries x 64 outputs per workgroup, a 4 x 4 tile per thread
+// (queries ty + 16r, outputs 4tx + c), with a 32-dim chunk of x and of M staged in LDS, so each M
+// element is read from memory once per 64 queries and each pair of LDS reads feeds 16 fmas. Every
+// output is still one fma chain over the input dims in order (bit-identical to k_rotate).
+constexpr uint32_t kRotBQ = 64, kRotBI = 64, kRotBJ = 32;
+__global__ __launch_bounds__(256) void k_rotate_tiled(const float* __restrict__ x, const float* __restrict__ M,
+                                                      float* __restrict__ y, uint8_t* __restrict__ yq, uint32_t d,
+                                                      uint32_t nq, float base, float scale) {
+    __shared__ float xs[kRotBQ][kRotBJ + 1];
+    __shared__ __align__(16) float ms[kRotBJ][kRotBI];
+    const uint32_t tx = threadIdx.x % 16, ty = threadIdx.x / 16;  // outputs 4tx..4tx+3, queries ty + 16r
+    const uint32_t i0 = blockIdx.x * kRotBI, q0 = blockIdx.y * kRotBQ;
+    float acc[4][4];
+#pragma unroll
+    for (uint32_t r = 0; r < 4; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) acc[r][c] = 0.0f;
+    for (uint32_t j0 = 0; j0 < d; j0 += kRotBJ) {
+        for (uint32_t t = threadIdx.x; t < kRotBQ * kRotBJ; t += 256) {
+            const uint32_t qq = t / kRotBJ, jj = t % kRotBJ, q = q0 + qq, j = j0 + jj;
+            xs[qq][jj] = (q < nq && j < d) ? x[(size_t)q * d + j] : 0.0f;
+        }
+        for (uint32_t t = threadIdx.x; t < kRotBJ * kRotBI; t += 256) {
+            const uint32_t jj = t / kRotBI, ii = t % kRotBI, j = j0 + jj, i = i0 + ii;
+            ms[jj][ii] = (j < d && i < d) ? M[(size_t)j * d + i] : 0.0f;
+        }
+        __syncthreads();
+        const uint32_t jl = min(kRotBJ, d - j0);  // only the real dims enter the chains
+        for (uint32_t jj = 0; jj < jl; ++jj) {
+            const float4 m = *reinterpret_cast<const float4*>(&ms[jj][4 * tx]);
+            float xv[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) xv[r] = xs[ty + 16 * r][jj];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                acc[r][0] = __builtin_fmaf(xv[r], m.x, acc[r][0]);
+                acc[r][1] = __builtin_fmaf(xv[r], m.y, acc[r][1]);
+                acc[r][2] = __builtin_fmaf(xv[r], m.z, acc[r][2]);
+                acc[r][3] = __builtin_fmaf(xv[r], m.w, acc[r][3]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < 4; ++r) {
+        const uint32_t q = q0 + ty + 16 * r;
+        if (q >= nq) continue;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+            const uint32_t i = i0 + 4 * tx + c;
+            if (i >= d) continue;
+            const float v = acc[r][c];
+            if (y) y[(size_t)q * d + i] = v;
+            if (yq) {
+                const float t = (v - base) * scale;
+                const int rr = static_cast<int>(roundf(t));
+                yq[(size_t)q * d + i] = static_cast<uint8_t>(rr > 255 ? 255 : (rr < 0 ? 0 : rr));
+            }
+        }
+    }
+}
+
 constexpr uint32_t kProbeCPB = 64;
 template <uint32_t QT>
 __global__ __launch_bounds__(256) void k_probe_dist(const float* __restrict__ qf, const float* __restrict__ cent,
