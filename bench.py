@@ -276,6 +276,20 @@ def main():
             break
         nprobe_q[:] = min(nlist, int(nprobe_q[0]) * 2)
     rec = sweep[key]
+    ceiling = None
+    if cfg["nprobe_range"] or cfg.get("nprobe_choices"):
+        # recall with every query probing 4x the configuration's largest nprobe: when it equals the
+        # recall above, the probe is not what limits recall (SQ8 rounding of close neighbours is)
+        saved = nprobe_q.copy()
+        big = min(nlist, 4 * int(saved.max()))
+        set_nprobe(np.full(NQ, big, np.uint32))
+        r_big = recall_now()
+        ceiling = {"nprobe": big, ("recall_at_k_per_query" if per_k else f"recall_at_{K}"): round(r_big, 4),
+                   "recall_at_10": round(state["recall10"], 4)}
+        log(f"recall at nprobe {big} for every query: {r_big:.4f} (recall@10 {state['recall10']:.4f})")
+        nprobe_q = saved
+        set_nprobe(nprobe_q)
+        recall_now()  # restores recall10 of the configuration's own nprobe
 
     # autotune the scan workgroup size per level on one batch ("waves per cluster")
     def tune(w, m):
@@ -428,6 +442,14 @@ def main():
 
     lat1g, nodes1 = graph_latency(ws1, 1, args.plain_queries)
     lat64g, _ = graph_latency(ws64, 64, 60)
+    # a whole batch through the persistent work queue (order 2), captured in one graph
+    wsb = state["ws"]
+    order_before = wsb.order
+    wsb.set_order(2)
+    latbg, nodesb = graph_latency(wsb, B, 10)
+    wsb.set_order(order_before)
+    log(f"graph-captured batch of {B} with the persistent work queue: {np.median(latbg):.3f} ms "
+        f"({B / np.median(latbg) * 1e3:.0f} QPS, {nodesb} nodes)")
     log(f"graph-captured: one query p50 {np.percentile(lat1g, 50):.3f} ms p99 {np.percentile(lat1g, 99):.3f} ms "
         f"({nodes1} nodes); batch-64 p50 {np.percentile(lat64g, 50):.3f} ms")
     log(f"plain per-query: {plain_qps:.0f} QPS p50 {np.percentile(lat1, 50):.3f} ms p99 {np.percentile(lat1, 99):.3f} ms; "
@@ -463,6 +485,7 @@ def main():
         ("recall_at_k_per_query" if per_k else f"recall_at_{K}"): round(rec, 4),
         "recall_at_10": round(state["recall10"], 4),
         "recall_sweep": sweep,
+        "recall_at_4x_max_nprobe": ceiling,
         "parity_vs_oracle": parity,
         "plain_per_query_qps": round(plain_qps, 1),
         "batched_over_plain": round(qps / plain_qps, 2),
@@ -473,6 +496,7 @@ def main():
                        "graph_one_query_p50": round(float(np.percentile(lat1g, 50)), 4),
                        "graph_one_query_p99": round(float(np.percentile(lat1g, 99)), 4),
                        "graph_batch64_p50": round(float(np.percentile(lat64g, 50)), 4),
+                       f"graph_batch{B}_persistent_queue_median": round(float(np.median(latbg)), 4),
                        f"batch{B}_mean": round(ms_per_step, 4)},
         "batch_service_ms": svc,
         "stage_ms": {"preprocess": round(float(stage[0]), 4), "probe": round(float(stage[1]), 4),
