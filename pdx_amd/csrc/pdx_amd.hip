@@ -234,7 +234,7 @@ struct Level {
 
 struct LdsPlan {
     uint32_t bytes = 0;
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_rat, o_ring;
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
 };
 
 LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
@@ -262,7 +262,6 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
     p.o_cp = take(block * 4);
     p.o_red = take(64);
     p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + 2 * kMaxCheckpoints + 1) * 4);
-    p.o_rat = take(L.d * 4);
     p.o_ring = take(kRing * sizeof(ClusterMeta));
     p.bytes = (off + 15) / 16 * 16;
     return p;
@@ -458,7 +457,6 @@ void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_
     a.o_cp = p.o_cp;
     a.o_red = p.o_red;
     a.o_tau = p.o_tau;
-    a.o_rat = p.o_rat;
     a.o_ring = p.o_ring;
     if constexpr (!FILT) {
         if (a.queue) {  // persistent: as many workgroups as are resident at once

@@ -65,7 +65,7 @@ struct ScanArgs {
     uint32_t* queue;            // optional work-queue counter (persistent launch): queries are
                                 // taken in perm order by atomically incrementing it
     // dynamic LDS layout (byte offsets)
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_rat, o_ring;
+    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
 };
 
 // Per-cluster parameters of the next kRing visited clusters, staged in LDS in one batch so that
@@ -410,7 +410,7 @@ struct Scanner {
     uint64_t *H, *T, *O, *CK;
     uint32_t *HP, *TP, *OP, *CP, *red;
     dist_t* taus;     // thresholds of the current prune steps
-    float* rat;       // ADSampling ratio table [d]
+    const float* rat;  // ADSampling ratio table [d] (global: read a few times per cluster, cached)
     ClusterMeta* ring;
     uint32_t* wcnt;   // SUMM: pruned counts per warmup checkpoint
     uint32_t* cpe;    // warmup checkpoint ends (cumulative fetch sizes clipped to v)
@@ -444,7 +444,7 @@ struct Scanner {
         CP = reinterpret_cast<uint32_t*>(lds + a.o_cp);
         red = reinterpret_cast<uint32_t*>(lds + a.o_red);
         taus = reinterpret_cast<dist_t*>(lds + a.o_tau);
-        rat = reinterpret_cast<float*>(lds + a.o_rat);
+        rat = a.ratios;
         ring = reinterpret_cast<ClusterMeta*>(lds + a.o_ring);
         d = a.L.d;
         v = a.L.v;
@@ -484,7 +484,6 @@ struct Scanner {
     __device__ __forceinline__ uint32_t qbyte(uint32_t j) const { return (qw[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
 
     __device__ void load_query(uint32_t q) {
-        for (uint32_t j = threadIdx.x; j < d; j += BLOCK) rat[j] = a.ratios[j];
         if (threadIdx.x == 0) {
             uint32_t e = 0;
             for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
