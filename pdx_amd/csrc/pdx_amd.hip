@@ -251,7 +251,7 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
     p.o_qq = take(L.u8 ? nw * 4 : 0);
     const uint32_t capn = L.max_used();
     p.o_dist = take(capn * 4);
-    p.o_pos = take(capn * 4);
+    p.o_pos = take(capn * 2);  // 16-bit slots
     p.o_h = take(kmax * 8);
     p.o_t = take(kmax * 8);
     p.o_o = take(kmax * 8);
@@ -439,6 +439,7 @@ void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
 
 template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false>
 void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
+    require(L.max_used() < kPosNone, "cluster too large for the scan's 16-bit slot lists", PDXG_ERR_UNSUPPORTED);
     LdsPlan p = plan_scan_lds(L, kmax, BLOCK);
     require(p.bytes <= (uint32_t)ix->max_lds,
             "cluster capacity / k too large for the on-chip scan buffers (" + std::to_string(p.bytes) + " B LDS)",

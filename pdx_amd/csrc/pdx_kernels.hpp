@@ -74,7 +74,8 @@ struct ClusterMeta {
     unsigned long long data_off, slot_off, tomb_off;
     uint32_t c, n_live, n_used, cap, n_tomb, aux;  // aux: passing slots (filtered) / summary offset
 };
-constexpr uint32_t kRing = 64;
+constexpr uint32_t kRing = 16;
+constexpr uint32_t kPosNone = 0xFFFFu;  // a survivor slot that was pruned
 
 // Largest number of warmup checkpoints (entries of DIMENSIONS_FETCHING_SIZES used).
 constexpr uint32_t kMaxCheckpoints = 20;
@@ -406,7 +407,7 @@ struct Scanner {
     uint32_t* qw;     // U8 query words: [nvw] vertical then [h/4] horizontal
     uint32_t* qq;     // U8 per-word sum of q^2
     dist_t* dist;
-    uint32_t* pos;
+    uint16_t* pos;    // survivor slots (a cluster's LDS distance array keeps it far below 2^16 slots)
     uint64_t *H, *T, *O, *CK;
     uint32_t *HP, *TP, *OP, *CP, *red;
     dist_t* taus;     // thresholds of the current prune steps
@@ -433,7 +434,7 @@ struct Scanner {
         qw = reinterpret_cast<uint32_t*>(lds + a.o_q);
         qq = reinterpret_cast<uint32_t*>(lds + a.o_qq);
         dist = reinterpret_cast<dist_t*>(lds + a.o_dist);
-        pos = reinterpret_cast<uint32_t*>(lds + a.o_pos);
+        pos = reinterpret_cast<uint16_t*>(lds + a.o_pos);
         H = reinterpret_cast<uint64_t*>(lds + a.o_h);
         HP = reinterpret_cast<uint32_t*>(lds + a.o_hp);
         T = reinterpret_cast<uint64_t*>(lds + a.o_t);
@@ -863,7 +864,7 @@ struct Scanner {
             bool p = i < n && selected(i);
             uint32_t s;
             uint32_t r = block_excl_scan<BLOCK>(p, red, &s);
-            if (p) pos[np + r] = i;
+            if (p) pos[np + r] = static_cast<uint16_t>(i);
             np += s;
         }
         __syncthreads();
@@ -1071,7 +1072,7 @@ struct Scanner {
             bool p = valid && x < tau && selected(i);
             uint32_t s;
             uint32_t r = block_excl_scan<BLOCK>(p, red, &s);
-            if (p) pos[np + r] = i;
+            if (p) pos[np + r] = static_cast<uint16_t>(i);
             np += s;
         }
         if (np == 0) return;
@@ -1105,7 +1106,7 @@ struct Scanner {
                 dist[i] = x;
                 ++alive_cnt;
             } else {
-                pos[j] = ~0u;
+                pos[j] = kPosNone;
             }
         }
         bytes_local += 4ull * alive_cnt;  // Merge reads one distance per survivor
@@ -1116,8 +1117,8 @@ struct Scanner {
                 hl = list_insert_chunk<BLOCK>(key[r] != kKeyInf, key[r], pay[r], H, HP, hl, k, CK, CP, O, OP, red);
         for (uint32_t j0 = kR * BLOCK; j0 < np; j0 += BLOCK) {
             const uint32_t j = j0 + threadIdx.x;
-            const uint32_t i = j < np ? pos[j] : ~0u;
-            const bool valid = i != ~0u;
+            const uint32_t i = j < np ? pos[j] : kPosNone;
+            const bool valid = i != kPosNone;
             hl = list_insert_chunk<BLOCK>(valid, valid ? key_of(i) : kKeyInf, valid ? pay_of(i) : 0u, H, HP, hl, k, CK,
                                           CP, O, OP, red);
         }
