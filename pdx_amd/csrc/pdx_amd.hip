@@ -602,6 +602,10 @@ void launch_preprocess(const pdxg_index* ix, pdxg_workspace& ws, const float* qu
         dim3 g((ix->d + kRotBI - 1) / kRotBI, (nq + kRotBQ - 1) / kRotBQ);
         hipLaunchKernelGGL(k_rotate_tiled, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p, yq, ix->d, nq,
                            ix->qbase, ix->qscale);
+    } else if (!transformed && nq <= kRotSmallQ && (size_t)ix->d * 4 <= 65536) {  // a few queries: wide
+        dim3 g((ix->d + 63) / 64, nq);
+        hipLaunchKernelGGL(k_rotate_small, g, dim3(64), ix->d * 4, st, src, ix->rotation.p, ws.qrot.p, yq, ix->d,
+                           ix->qbase, ix->qscale);
     } else {
         dim3 g((ix->d + kRotTI - 1) / kRotTI, (nq + kRotTQ - 1) / kRotTQ);
         hipLaunchKernelGGL(k_rotate, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p, yq, ix->d, nq, ix->qbase,

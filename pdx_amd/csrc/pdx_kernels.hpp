@@ -261,6 +261,38 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ x, con
 // dist[q][c] = l2sq16(q, centroid c). One workgroup per (QT queries, 64 centroids): the QT queries
 // sit in LDS and each 16-lane row streams one centroid at a time, reusing every centroid load for
 // all QT queries (QT accumulators per lane, each in the 16-lane order of l2sq16).
+// Rotation of a few queries (nq <= kRotSmallQ): one workgroup of 64 threads per (64 outputs, query),
+// the query staged in LDS (d floats, dynamic), each thread one output's fma chain over the input
+// dims in order with the column loads of M issued 8 ahead. Bit-identical to k_rotate; it spreads a
+// single query over d / 64 workgroups instead of leaving each chain behind a dependent load.
+constexpr uint32_t kRotSmallQ = 8;
+__global__ __launch_bounds__(64) void k_rotate_small(const float* __restrict__ x, const float* __restrict__ M,
+                                                     float* __restrict__ y, uint8_t* __restrict__ yq, uint32_t d,
+                                                     float base, float scale) {
+    extern __shared__ float xq[];  // [d]
+    const uint32_t q = blockIdx.y, i = blockIdx.x * 64 + threadIdx.x;
+    for (uint32_t j = threadIdx.x; j < d; j += 64) xq[j] = x[(size_t)q * d + j];
+    __syncthreads();
+    if (i >= d) return;
+    const float* col = M + i;
+    float acc = 0.0f;
+    uint32_t j = 0;
+    for (; j + 8 <= d; j += 8) {
+        float m[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) m[u] = col[(size_t)(j + u) * d];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) acc = __builtin_fmaf(xq[j + u], m[u], acc);
+    }
+    for (; j < d; ++j) acc = __builtin_fmaf(xq[j], col[(size_t)j * d], acc);
+    if (y) y[(size_t)q * d + i] = acc;
+    if (yq) {
+        const float t = (acc - base) * scale;
+        const int rr = static_cast<int>(roundf(t));
+        yq[(size_t)q * d + i] = static_cast<uint8_t>(rr > 255 ? 255 : (rr < 0 ? 0 : rr));
+    }
+}
+
 // Batched rotation (nq >= kRotBQ): 64 queries x 64 outputs per workgroup, a 4 x 4 tile per thread
 // (queries ty + 16r, outputs 4tx + c), with a 32-dim chunk of x and of M staged in LDS, so each M
 // element is read from memory once per 64 queries and each pair of LDS reads feeds 16 fmas. Every

@@ -84,6 +84,9 @@ def test_preprocess_matches_oracle(gpu_lib, case):
     rot_b, qu_b = g.preprocess(qb)
     np.testing.assert_array_equal(rot_b.view(np.uint32), o.preprocess(qb).view(np.uint32))
     np.testing.assert_array_equal(rot_b[:len(q)].view(np.uint32), g.preprocess(qb[:len(q)])[0].view(np.uint32))
+    # one and a few queries (the wide small-batch rotation)
+    for m in (1, 5, 8):
+        np.testing.assert_array_equal(g.preprocess(qb[:m])[0].view(np.uint32), rot_b[:m].view(np.uint32))
 
 
 def test_mixed_k_and_nprobe_per_query(gpu_lib):
