@@ -550,8 +550,16 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(1024), 0, st, m);
     hipLaunchKernelGGL(k_meso_scatter, gp, dim3(256), 0, st, m);
     hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(256), 0, st, m);
-    // work items: sum over clusters of ceil(entries / kMesoQ) <= pairs / kMesoQ + clusters
-    const size_t items = pairs / kMesoQ + L.nc;
+    // work items: sum over clusters of ceil(entries / kMesoQ) x chunks, where a cluster's entries
+    // are at most nq and all entries at most `pairs`
+    size_t chunks = 0, ch_max = 1;
+    for (uint32_t c = 0; c < L.nc; ++c) {
+        const size_t ch = std::max<size_t>(1, (L.n_used[c] + kWave - 1) / kWave);
+        chunks += ch;
+        ch_max = std::max(ch_max, ch);
+    }
+    const size_t items = std::min(((pairs + kMesoQ - 1) / kMesoQ + L.nc) * ch_max,
+                                  ((size_t)nq + kMesoQ - 1) / kMesoQ * chunks);
     const size_t wgs = (items + kMesoWaves - 1) / kMesoWaves;
     m.wg_per_xcd = (uint32_t)((wgs + 7) / 8);
     require(8ull * m.wg_per_xcd * kMesoWaves * kWave < (1ull << 32), "too many meso work items for one dispatch",
@@ -598,7 +606,7 @@ void launch_preprocess(const pdxg_index* ix, pdxg_workspace& ws, const float* qu
         src = ws.qnorm.p;
     }
     uint8_t* yq = ix->u8() ? ws.qu8.p : static_cast<uint8_t*>(nullptr);
-    if (!transformed && nq >= kRotBQ) {  // batches: register-blocked tiles
+    if (!transformed && nq >= kRotTiledMin) {  // large batches: register-blocked tiles
         dim3 g((ix->d + kRotBI - 1) / kRotBI, (nq + kRotBQ - 1) / kRotBQ);
         hipLaunchKernelGGL(k_rotate_tiled, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p, yq, ix->d, nq,
                            ix->qbase, ix->qscale);

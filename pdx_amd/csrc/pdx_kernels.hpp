@@ -293,11 +293,14 @@ __global__ __launch_bounds__(64) void k_rotate_small(const float* __restrict__ x
     }
 }
 
-// Batched rotation (nq >= kRotBQ): 64 queries x 64 outputs per workgroup, a 4 x 4 tile per thread
+// Batched rotation (nq >= kRotTiledMin): 64 queries x 64 outputs per workgroup, a 4 x 4 tile per thread
 // (queries ty + 16r, outputs 4tx + c), with a 32-dim chunk of x and of M staged in LDS, so each M
 // element is read from memory once per 64 queries and each pair of LDS reads feeds 16 fmas. Every
 // output is still one fma chain over the input dims in order (bit-identical to k_rotate).
 constexpr uint32_t kRotBQ = 64, kRotBI = 64, kRotBJ = 32;
+// below this many queries the 16-query kernel spreads better (measured: 64 queries 124 us vs 227 us
+// tiled; 4096 queries 316 us vs 273 us tiled)
+constexpr uint32_t kRotTiledMin = 2048;
 __global__ __launch_bounds__(256) void k_rotate_tiled(const float* __restrict__ x, const float* __restrict__ M,
                                                       float* __restrict__ y, uint8_t* __restrict__ yq, uint32_t d,
                                                       uint32_t nq, float base, float scale) {

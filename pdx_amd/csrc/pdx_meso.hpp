@@ -9,9 +9,9 @@
 //            compute, for every visited (query, vector) pair, every partial distance the scan can
 //            compare: the vertical fma chain at each warmup checkpoint (the cumulative
 //            DIMENSIONS_FETCHING_SIZES clipped to v, common.hpp:47-49) and each 64-dim horizontal
-//            block distance in the 16-lane order. A work item is (cluster, 8 entries) on one
-//            wave; the 8 queries are wave-uniform and read through the scalar cache, the cluster's
-//            vectors map one per lane (coalesced vertical reads).
+//            block distance in the 16-lane order. A work item is (cluster, 8 entries, 64-vector
+//            chunk) on one wave; the 8 queries are wave-uniform and read through the scalar cache,
+//            the chunk's vectors map one per lane (coalesced vertical reads).
 //   phase 2  k_scan<..., SUMM = true>: the unchanged Start / Warmup / Prune / Merge control flow
 //            per query, reading those values instead of the vectors.
 //
@@ -34,7 +34,8 @@ struct MesoArgs {
     uint32_t* cnt;              // [nc] entries per cluster
     uint32_t* cur;              // [nc] scatter cursors
     uint32_t* lstart;           // [nc + 1] first entry of each cluster
-    uint32_t* cstart;           // [nc + 1] first work item of each cluster
+    uint32_t* cstart;           // [nc + 1] first work item of each cluster; a cluster's items are
+                                // (8-entry group, 64-vector chunk) pairs, chunk-minor
     uint32_t* list;             // entries q * stride + rank, grouped by cluster
     uint32_t* soff;             // [nq x stride] first summarised vector of (query, rank) in its region
     float* summ;                // per query: `region` vectors x ns floats
@@ -66,7 +67,8 @@ __global__ __launch_bounds__(1024) void k_meso_offsets(MesoArgs a) {
         const uint32_t c = m < nc ? a.cnt[m] : 0u;
         uint32_t lt, ct;
         const uint32_t lx = block_excl_sum<1024>(c, red, &lt);
-        const uint32_t cx = block_excl_sum<1024>((c + kMesoQ - 1) / kMesoQ, red, &ct);
+        const uint32_t nch = m < nc ? max(1u, (a.L.n_used[m] + kWave - 1) / kWave) : 0u;
+        const uint32_t cx = block_excl_sum<1024>((c + kMesoQ - 1) / kMesoQ * nch, red, &ct);
         if (m < nc) {
             a.lstart[m] = lbase + lx;
             a.cstart[m] = cbase + cx;
@@ -121,7 +123,7 @@ __device__ __forceinline__ const_float* to_const(const float* p) {
 template <uint32_t NQ>
 __device__ __forceinline__ void meso_summarize(const MesoArgs& a, const float* data, uint32_t S, uint32_t n,
                                                const_float* qbase, const uint32_t (&qo)[kMesoQ],
-                                               const uint32_t (&ob)[kMesoQ]) {
+                                               const uint32_t (&ob)[kMesoQ], uint32_t i_begin) {
     const uint32_t lane = threadIdx.x % kWave;
     const uint32_t v = a.L.v, nh = a.L.h / kHDim;
     const bool aligned = ((size_t)v * S) % 4 == 0;  // 16-byte aligned horizontal blocks
@@ -130,7 +132,8 @@ __device__ __forceinline__ void meso_summarize(const MesoArgs& a, const float* d
     const_float* qp[NQ];
 #pragma unroll
     for (uint32_t u = 0; u < NQ; ++u) qp[u] = qbase + qo[u];
-    for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
+    {  // one 64-vector chunk per work item
+        const uint32_t i0 = i_begin;
         const uint32_t i = i0 + lane;
         const bool valid = i < n;
         const uint32_t ii = valid ? i : n - 1;
@@ -234,7 +237,10 @@ __global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
         else hi = mid;
     }
     const uint32_t m = lo;
-    const uint32_t e0 = a.lstart[m] + (item - a.cstart[m]) * kMesoQ;
+    const uint32_t nch = max(1u, (a.L.n_used[m] + kWave - 1) / kWave);
+    const uint32_t local = item - a.cstart[m];
+    const uint32_t e0 = a.lstart[m] + (local / nch) * kMesoQ;
+    const uint32_t i_begin = (local % nch) * kWave;
     const uint32_t ne = min(kMesoQ, a.lstart[m + 1] - e0);
     const uint32_t n = a.L.n_used[m];
     if (n == 0) return;
@@ -255,14 +261,14 @@ __global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
         ob[u] = (ok && (unsigned long long)so + n <= a.region) ? static_cast<uint32_t>(q * a.region + so) : ~0u;
     }
     switch (ne) {  // the last item of a cluster may hold fewer queries
-        case 8: meso_summarize<8>(a, data, S, n, qbase, qo, ob); break;
-        case 7: meso_summarize<7>(a, data, S, n, qbase, qo, ob); break;
-        case 6: meso_summarize<6>(a, data, S, n, qbase, qo, ob); break;
-        case 5: meso_summarize<5>(a, data, S, n, qbase, qo, ob); break;
-        case 4: meso_summarize<4>(a, data, S, n, qbase, qo, ob); break;
-        case 3: meso_summarize<3>(a, data, S, n, qbase, qo, ob); break;
-        case 2: meso_summarize<2>(a, data, S, n, qbase, qo, ob); break;
-        default: meso_summarize<1>(a, data, S, n, qbase, qo, ob); break;
+        case 8: meso_summarize<8>(a, data, S, n, qbase, qo, ob, i_begin); break;
+        case 7: meso_summarize<7>(a, data, S, n, qbase, qo, ob, i_begin); break;
+        case 6: meso_summarize<6>(a, data, S, n, qbase, qo, ob, i_begin); break;
+        case 5: meso_summarize<5>(a, data, S, n, qbase, qo, ob, i_begin); break;
+        case 4: meso_summarize<4>(a, data, S, n, qbase, qo, ob, i_begin); break;
+        case 3: meso_summarize<3>(a, data, S, n, qbase, qo, ob, i_begin); break;
+        case 2: meso_summarize<2>(a, data, S, n, qbase, qo, ob, i_begin); break;
+        default: meso_summarize<1>(a, data, S, n, qbase, qo, ob, i_begin); break;
     }
 }
 
