@@ -124,6 +124,13 @@ __global__ __launch_bounds__(256) void k_dense_convert(const float* __restrict__
     }
 }
 
+// BF16 copy of n floats (grid-stride).
+__global__ void k_to_bf16(const float* __restrict__ src, unsigned long long n, __hip_bfloat16* __restrict__ dst) {
+    const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step)
+        dst[e] = __float2bfloat16(src[e]);
+}
+
 // FP32 squared norm of each centroid (FP64 accumulation), one workgroup per centroid.
 __global__ __launch_bounds__(256) void k_dense_cnorm(const float* cent, uint32_t d, float* cn) {
     __shared__ double red[4];
@@ -136,8 +143,10 @@ __global__ __launch_bounds__(256) void k_dense_cnorm(const float* cent, uint32_t
     if (threadIdx.x == 0) cn[blockIdx.x] = static_cast<float>(t);
 }
 
-// Cluster of each appended row: argmin_c ||c||^2 - 2 x.c (G holds -2 x.c, column-major: column
-// t = row t over the nc centroids); ties go to the smaller cluster. One wave per row.
+// Cluster of each appended row: argmin_c ||c||^2 - 2 x.c (G holds -2 x.c from a BF16 GEMM,
+// column-major: column t = row t over the nc centroids); ties go to the smaller cluster. One wave
+// per row. The anchor only sets which residual is coded: any centroid gives the same exact
+// results, a near one a small residual.
 __global__ __launch_bounds__(64) void k_dense_anchor(const float* __restrict__ G, const float* __restrict__ cn,
                                                      uint32_t nc, unsigned long long r0, uint32_t* rcl) {
     const uint32_t t = blockIdx.x, lane = threadIdx.x;

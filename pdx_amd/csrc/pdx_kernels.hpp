@@ -298,9 +298,9 @@ __global__ __launch_bounds__(64) void k_rotate_small(const float* __restrict__ x
 // element is read from memory once per 64 queries and each pair of LDS reads feeds 16 fmas. Every
 // output is still one fma chain over the input dims in order (bit-identical to k_rotate).
 constexpr uint32_t kRotBQ = 64, kRotBI = 64, kRotBJ = 32;
-// below this many queries the 16-query kernel spreads better (measured: 64 queries 124 us vs 227 us
-// tiled; 4096 queries 316 us vs 273 us tiled)
-constexpr uint32_t kRotTiledMin = 2048;
+// below this many queries the 16-query kernel spreads better (measured at d = 1024: 64 queries
+// 124 us vs 227 us tiled; 4096 queries 316 us vs 273 us tiled)
+constexpr uint32_t kRotTiledMin = 1024;
 __global__ __launch_bounds__(256) void k_rotate_tiled(const float* __restrict__ x, const float* __restrict__ M,
                                                       float* __restrict__ y, uint8_t* __restrict__ yq, uint32_t d,
                                                       uint32_t nq, float base, float scale) {
