@@ -351,7 +351,8 @@ def main():
     if tree and pairs > 0:
         flops = pairs * (3 * cfg["d"] + 15 * (idx.l1.h // 64))
         tf = flops / (meso[0] * 1e-3) / 1e12
-        roofs["meso_phase1"] = {"bound": "valu", "kernel": "k_meso_summary (two-phase meso scan, phase 1)",
+        roofs["meso_phase1"] = {"bound": "valu",
+                                "kernel": "k_meso_summary + its visit-list kernels (two-phase meso scan, phase 1)",
                                 "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                 "frac": round(tf / FP32_PEAK_TFLOPS, 4), "ms": round(float(meso[0]), 4),
                                 "pairs_per_launch": int(pairs), "flops_per_launch": int(flops)}

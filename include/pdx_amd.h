@@ -200,7 +200,8 @@ pdxg_status pdxg_search_meso(pdxg_index* index, const float* queries_transformed
  * query per call on one thread). Requests (one query each, own k and nprobe) are queued by
  * pdxg_server_submit; a dispatcher thread packs the waiting requests into a batch whenever one of
  * the server's slots is free and launches it on that slot's stream, so batches overlap on the
- * device; a completion thread returns results in completion order. The number of slots follows
+ * device; a completion thread polls the batches' events in dispatch order and hands a batch's
+ * results back once it and every batch dispatched before it are done. The number of slots follows
  * from the scratch budget: each slot owns one workspace of max_batch x max_k x max_nprobe
  * (pdxg_workspace_bytes), so admission is bounded by scratch memory. Per-query results do not
  * depend on what a query is batched with.
