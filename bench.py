@@ -482,7 +482,8 @@ def main():
         "config": {"workload": workload, "global_batch": B, "parallelism": "single GPU",
                    "scan_block": {"batch": list(plan), "batch64": list(plan64), "one_query": list(plan1)},
                    "meso_two_phase": {"batch": ws.meso_summary, "batch64": ws64.meso_summary,
-                                      "one_query": ws1.meso_summary} if tree else None},
+                                      "one_query": ws1.meso_summary} if tree else None,
+                   "scan_order": {"batch": ws.order, "batch64": ws64.order, "one_query": ws1.order}},
         ("recall_at_k_per_query" if per_k else f"recall_at_{K}"): round(rec, 4),
         "recall_at_10": round(state["recall10"], 4),
         "recall_sweep": sweep,

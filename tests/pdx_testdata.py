@@ -41,3 +41,20 @@ def index_file(kind: str, n: int, d: int, nlist: int, seed: int = 7, normalize: 
         pf.save(idx, path)
         _CACHE[key] = (path, x, q)
     return _CACHE[key]
+
+
+def c1_index():
+    """The reference's CPU configuration: 65,536 x 128 from a 64-component mixture (centres N(0, I),
+    sigma 0.25), 256 IVF lists, 100 queries from the mixture plus N(0, 0.05^2) noise."""
+    key = "c1"
+    if key not in _CACHE:
+        rng = np.random.default_rng(65536)
+        c = rng.standard_normal((64, 128)).astype(np.float32)
+        x = (c[rng.integers(0, 64, 65536)] + 0.25 * rng.standard_normal((65536, 128))).astype(np.float32)
+        q = (c[rng.integers(0, 64, 100)] + 0.25 * rng.standard_normal((100, 128))
+             + 0.05 * rng.standard_normal((100, 128))).astype(np.float32)
+        idx = builder.build_index(x, "pdx_f32", num_clusters=256, seed=42, kmeans_iters=10, device=False)
+        path = os.path.join(_DIR, "c1.pdx")
+        pf.save(idx, path)
+        _CACHE[key] = (path, x, q)
+    return _CACHE[key]

@@ -191,3 +191,20 @@ def test_two_phase_meso_scan_matches_pruned_and_oracle(gpu_lib, kind, n, d, nlis
             res = (ids.cpu().numpy().view(np.uint32), dd.cpu().numpy(), idd.cpu().numpy().view(np.uint32),
                    cnt.cpu().numpy().view(np.uint32), nb.cpu().numpy().view(np.uint64))
             _compare((oi, od, oid, oc, ob), res, o.is_u8)
+
+
+def test_c1_configuration_one_query_at_a_time(gpu_lib):
+    """The reference's CPU case (IVF F32, 65,536 x 128, 256 lists, k = 10, nprobe = 16), each query
+    searched alone as the reference does, bit-exact with the oracle."""
+    from tests.pdx_testdata import c1_index
+
+    path, x, q = c1_index()
+    o = OracleIndex(path)
+    g = gpu_lib.GpuIndex(pf.load(path))
+    oi, od, oid, oc, ob = o.search(q, 10, 16)
+    for t in range(len(q)):
+        gi, gd, gid, gc, gb = g.search(q[t:t + 1], 10, 16, want_bytes=True)
+        np.testing.assert_array_equal(gc[0], oc[t])
+        np.testing.assert_array_equal(gi[0, :oc[t]], oi[t, :oc[t]])
+        np.testing.assert_array_equal(gd[0, :oc[t]].view(np.uint32), od[t, :oc[t]].view(np.uint32))
+        np.testing.assert_array_equal(gb[0], ob[t])

@@ -309,3 +309,17 @@ def test_rotation_is_the_matrix_product_and_preserves_norms():
     # a transposed matrix is a different map: the test would catch y = M . x
     wrong = q.astype(np.float64) @ idx.matrix.astype(np.float64).T
     assert np.abs(wrong - ref).max() > 1e-2
+
+
+def test_c1_configuration_recall():
+    """The reference's CPU case (IVF F32, 65,536 x 128, 256 lists, k = 10, nprobe = 16, one query at
+    a time): the oracle's answers against FP64 brute force."""
+    from tests.pdx_testdata import c1_index
+
+    path, x, q = c1_index()
+    o = OracleIndex(path)
+    ids, dist, _, cnt, _ = o.search(q, 10, 16)
+    x64 = x.astype(np.float64)
+    gt = [np.argsort(((x64 - qq.astype(np.float64)) ** 2).sum(1), kind="stable")[:10] for qq in q]
+    rec = np.mean([len(set(ids[i, :cnt[i]].tolist()) & set(gt[i].tolist())) / 10 for i in range(len(q))])
+    assert rec >= 0.99
