@@ -545,10 +545,11 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     require((uint64_t)nq * ws.order_stride < (1ull << 32), "too many queries x meso clusters for one search");
     hipLaunchKernelGGL(k_fill_u32, dim3((L.nc + 255) / 256), dim3(256), 0, st, ws.m_cnt.p, L.nc, 0u);
     const size_t pairs = (size_t)nq * ws.nv0_max;
-    const dim3 gp((unsigned)((pairs + 255) / 256));
-    hipLaunchKernelGGL(k_meso_count, gp, dim3(256), 0, st, m);
+    const dim3 gp((unsigned)((pairs + kListPer - 1) / kListPer));
+    require(L.nc <= kListHistMax, "too many meso clusters for the visit-list histogram", PDXG_ERR_UNSUPPORTED);
+    hipLaunchKernelGGL(k_meso_count, gp, dim3(kListThreads), L.nc * 4, st, m);
     hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(1024), 0, st, m);
-    hipLaunchKernelGGL(k_meso_scatter, gp, dim3(256), 0, st, m);
+    hipLaunchKernelGGL(k_meso_scatter, gp, dim3(kListThreads), L.nc * 4, st, m);
     hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(256), 0, st, m);
     // work items: sum over clusters of ceil(entries / kMesoQ) x chunks, where a cluster's entries
     // are at most nq and all entries at most `pairs`

@@ -50,11 +50,32 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_meso_count(MesoArgs a) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)a.nq * a.nv_max) return;
-    const uint32_t q = static_cast<uint32_t>(t / a.nv_max), r = static_cast<uint32_t>(t % a.nv_max);
-    if (r < a.nvisit[q]) atomicAdd(&a.cnt[a.order[(size_t)q * a.stride + r]], 1u);
+// Visit-list inversion. A workgroup takes kListPer consecutive (query, rank) entries; the
+// per-cluster counts go to an LDS histogram ([nc] words of dynamic LDS) and reach the global
+// counters with one atomic per (workgroup, cluster), instead of one per entry on a few hundred
+// contended addresses.
+constexpr uint32_t kListThreads = 256;
+constexpr uint32_t kListPerThread = 8;
+constexpr uint32_t kListPer = kListThreads * kListPerThread;
+constexpr uint32_t kListHistMax = 16384;  // largest nc for the LDS histogram (64 KB); the meso probe allows 8192
+
+__global__ __launch_bounds__(kListThreads) void k_meso_count(MesoArgs a) {
+    extern __shared__ uint32_t hist[];
+    const uint32_t nc = a.L.nc;
+    const size_t total = (size_t)a.nq * a.nv_max;
+    const size_t e0 = (size_t)blockIdx.x * kListPer;
+    for (uint32_t i = threadIdx.x; i < nc; i += kListThreads) hist[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kListPerThread; ++k) {
+        const size_t t = e0 + k * kListThreads + threadIdx.x;
+        if (t >= total) break;
+        const uint32_t q = static_cast<uint32_t>(t / a.nv_max), r = static_cast<uint32_t>(t % a.nv_max);
+        if (r < a.nvisit[q]) atomicAdd(&hist[a.order[(size_t)q * a.stride + r]], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nc; i += kListThreads)
+        if (hist[i]) atomicAdd(&a.cnt[i], hist[i]);
 }
 
 // Entry and work-item offsets per cluster (one workgroup of 1024).
@@ -83,13 +104,35 @@ __global__ __launch_bounds__(1024) void k_meso_offsets(MesoArgs a) {
     }
 }
 
-__global__ void k_meso_scatter(MesoArgs a) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)a.nq * a.nv_max) return;
-    const uint32_t q = static_cast<uint32_t>(t / a.nv_max), r = static_cast<uint32_t>(t % a.nv_max);
-    if (r >= a.nvisit[q]) return;
-    const uint32_t m = a.order[(size_t)q * a.stride + r];
-    a.list[a.lstart[m] + atomicAdd(&a.cur[m], 1u)] = q * a.stride + r;
+// Entries land in their cluster's list in no particular order (each entry names its query and
+// rank, and phase 1 writes each value to that query's own summary region). A workgroup ranks its
+// entries in LDS and reserves one range per cluster with a single global atomic.
+__global__ __launch_bounds__(kListThreads) void k_meso_scatter(MesoArgs a) {
+    extern __shared__ uint32_t hist[];
+    const uint32_t nc = a.L.nc;
+    const size_t total = (size_t)a.nq * a.nv_max;
+    const size_t e0 = (size_t)blockIdx.x * kListPer;
+    for (uint32_t i = threadIdx.x; i < nc; i += kListThreads) hist[i] = 0;
+    __syncthreads();
+    uint32_t mm[kListPerThread], val[kListPerThread], rank[kListPerThread];
+#pragma unroll
+    for (uint32_t k = 0; k < kListPerThread; ++k) {
+        mm[k] = 0xFFFFFFFFu;
+        const size_t t = e0 + k * kListThreads + threadIdx.x;
+        if (t >= total) continue;
+        const uint32_t q = static_cast<uint32_t>(t / a.nv_max), r = static_cast<uint32_t>(t % a.nv_max);
+        if (r >= a.nvisit[q]) continue;
+        mm[k] = a.order[(size_t)q * a.stride + r];
+        val[k] = q * a.stride + r;
+        rank[k] = atomicAdd(&hist[mm[k]], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nc; i += kListThreads)
+        if (hist[i]) hist[i] = atomicAdd(&a.cur[i], hist[i]);
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kListPerThread; ++k)
+        if (mm[k] != 0xFFFFFFFFu) a.list[a.lstart[mm[k]] + hist[mm[k]] + rank[k]] = val[k];
 }
 
 // Summary offsets of a query's visits: running sum of the visited clusters' sizes.
