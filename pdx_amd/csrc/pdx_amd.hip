@@ -234,10 +234,10 @@ struct Level {
 
 struct LdsPlan {
     uint32_t bytes = 0;
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
+    uint32_t o_q, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
 };
 
-LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
+LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block, bool summ) {
     LdsPlan p{};
     uint32_t off = 0;
     auto take = [&](uint32_t bytes) {
@@ -248,7 +248,6 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
     };
     const uint32_t nw = (L.v + 3) / 4 + L.h / 4;
     p.o_q = take(L.u8 ? nw * 4 : L.d * 4);
-    p.o_qq = take(L.u8 ? nw * 4 : 0);
     const uint32_t capn = L.max_used();
     p.o_dist = take(capn * 4);
     p.o_pos = take(capn * 2);  // 16-bit slots
@@ -261,7 +260,8 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block) {
     p.o_op = take(kmax * 4);
     p.o_cp = take(block * 4);
     p.o_red = take(64);
-    p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + 2 * kMaxCheckpoints + 1) * 4);
+    // block thresholds; the replay adds its checkpoint counts and ends
+    p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + (summ ? 2 * kMaxCheckpoints + 1 : 0)) * 4);
     p.o_ring = take(kRing * sizeof(ClusterMeta));
     p.bytes = (off + 15) / 16 * 16;
     return p;
@@ -440,12 +440,11 @@ void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
 template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false>
 void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax, hipStream_t st) {
     require(L.max_used() < kPosNone, "cluster too large for the scan's 16-bit slot lists", PDXG_ERR_UNSUPPORTED);
-    LdsPlan p = plan_scan_lds(L, kmax, BLOCK);
+    LdsPlan p = plan_scan_lds(L, kmax, BLOCK, SUMM);
     require(p.bytes <= (uint32_t)ix->max_lds,
             "cluster capacity / k too large for the on-chip scan buffers (" + std::to_string(p.bytes) + " B LDS)",
             PDXG_ERR_UNSUPPORTED);
     a.o_q = p.o_q;
-    a.o_qq = p.o_qq;
     a.o_dist = p.o_dist;
     a.o_pos = p.o_pos;
     a.o_h = p.o_h;

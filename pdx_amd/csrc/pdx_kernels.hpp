@@ -65,7 +65,7 @@ struct ScanArgs {
     uint32_t* queue;            // optional work-queue counter (persistent launch): queries are
                                 // taken in perm order by atomically incrementing it
     // dynamic LDS layout (byte offsets)
-    uint32_t o_q, o_qq, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
+    uint32_t o_q, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
 };
 
 // Per-cluster parameters of the next kRing visited clusters, staged in LDS in one batch so that
@@ -440,7 +440,6 @@ struct Scanner {
     uint8_t* lds;
     float* qf;        // F32 query [d]
     uint32_t* qw;     // U8 query words: [nvw] vertical then [h/4] horizontal
-    uint32_t* qq;     // U8 per-word sum of q^2
     dist_t* dist;
     uint16_t* pos;    // survivor slots (a cluster's LDS distance array keeps it far below 2^16 slots)
     uint64_t *H, *T, *O, *CK;
@@ -467,7 +466,6 @@ struct Scanner {
     __device__ Scanner(const ScanArgs& args, uint8_t* l) : a(args), lds(l) {
         qf = reinterpret_cast<float*>(lds + a.o_q);
         qw = reinterpret_cast<uint32_t*>(lds + a.o_q);
-        qq = reinterpret_cast<uint32_t*>(lds + a.o_qq);
         dist = reinterpret_cast<dist_t*>(lds + a.o_dist);
         pos = reinterpret_cast<uint16_t*>(lds + a.o_pos);
         H = reinterpret_cast<uint64_t*>(lds + a.o_h);
@@ -520,7 +518,7 @@ struct Scanner {
     __device__ __forceinline__ uint32_t qbyte(uint32_t j) const { return (qw[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
 
     __device__ void load_query(uint32_t q) {
-        if (threadIdx.x == 0) {
+        if (SUMM && threadIdx.x == 0) {
             uint32_t e = 0;
             for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
                 e = min(e + c_fetch[t], v);
@@ -538,13 +536,11 @@ struct Scanner {
                     word |= (j < v ? static_cast<uint32_t>(src[j]) : 0u) << (8 * t);
                 }
                 qw[w] = word;
-                qq[w] = sumsq4(word);
             }
             for (uint32_t w = threadIdx.x; w < h / 4; w += BLOCK) {
                 uint32_t word = 0;
                 for (uint32_t t = 0; t < 4; ++t) word |= static_cast<uint32_t>(src[v + 4 * w + t]) << (8 * t);
                 qw[nvw + w] = word;
-                qq[nvw + w] = sumsq4(word);
             }
         }
         __syncthreads();
@@ -613,7 +609,7 @@ struct Scanner {
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u) {
                     const uint32_t w = (j >> 2) + u;
-                    const uint32_t qv = qw[w], qs = qq[w];
+                    const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
                     for (uint32_t t = 0; t < kVT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
                 }
@@ -628,14 +624,14 @@ struct Scanner {
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     const uint32_t w = (j >> 2) + u;
-                    const uint32_t qv = qw[w], qs = qq[w];
+                    const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
                     for (uint32_t t = 0; t < kVT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
                 }
             }
             for (; j + 4 <= db; j += 4) {
                 const uint32_t w = j >> 2;
-                const uint32_t qv = qw[w], qs = qq[w];
+                const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
                 for (uint32_t t = 0; t < kVT; ++t)
                     acc[t] += sqdist4(qv, *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * idx[t]), qs);
@@ -665,7 +661,7 @@ struct Scanner {
             uint32_t j = da;
             for (; j + 4 <= db; j += 4) {
                 const uint32_t w = j >> 2;
-                acc += sqdist4(qw[w], *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * i), qq[w]);
+                acc += sqdist4(qw[w], *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * i), sumsq4(qw[w]));
             }
             if (j < db) {
                 const uint32_t rem = db - j;
@@ -699,7 +695,7 @@ struct Scanner {
             const uint8_t* blk = base + (size_t)(v + hd) * S + (size_t)i * kHDim;
             const uint32_t w = nvw + hd / 4 + l;
             uint32_t x = *reinterpret_cast<const uint32_t*>(blk + 4 * l);
-            return sum16_u(sqdist4(qw[w], x, qq[w]));
+            return sum16_u(sqdist4(qw[w], x, sumsq4(qw[w])));
         }
     }
 
@@ -792,7 +788,7 @@ struct Scanner {
         const uint32_t w0 = nvw + hd / 4;
         uint32_t acc = 0;
 #pragma unroll
-        for (uint32_t w = 0; w < 16; ++w) acc += sqdist4(qw[w0 + w], x[w], qq[w0 + w]);
+        for (uint32_t w = 0; w < 16; ++w) acc += sqdist4(qw[w0 + w], x[w], sumsq4(qw[w0 + w]));
         return acc;
     }
 
