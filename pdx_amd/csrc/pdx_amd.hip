@@ -549,7 +549,7 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     hipLaunchKernelGGL(k_meso_count, gp, dim3(kListThreads), L.nc * 4, st, m);
     hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(1024), 0, st, m);
     hipLaunchKernelGGL(k_meso_scatter, gp, dim3(kListThreads), L.nc * 4, st, m);
-    hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(256), 0, st, m);
+    hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(kWave), 0, st, m);
     // work items: sum over clusters of ceil(entries / kMesoQ) x chunks, where a cluster's entries
     // are at most nq and all entries at most `pairs`
     size_t chunks = 0, ch_max = 1;

@@ -192,8 +192,17 @@ __global__ void k_normalize(const float* __restrict__ x, float* __restrict__ out
     if (q >= nq) return;
     const float* xq = x + (size_t)q * d;
     if (threadIdx.x < 16) {
+        // lane l sums dims l, l + 16, ... in that order; 8 loads are in flight ahead of their fmas
         float acc = 0.0f;
-        for (uint32_t j = threadIdx.x; j < d; j += 16) acc = __builtin_fmaf(xq[j], xq[j], acc);
+        uint32_t j = threadIdx.x;
+        for (; j + 7 * 16 < d; j += 8 * 16) {
+            float v[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) v[u] = xq[j + 16 * u];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) acc = __builtin_fmaf(v[u], v[u], acc);
+        }
+        for (; j < d; j += 16) acc = __builtin_fmaf(xq[j], xq[j], acc);
         acc = tree16(acc);
         if (threadIdx.x == 0) {
             s_zero = acc == 0.0f;
@@ -263,7 +272,7 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ x, con
 // all QT queries (QT accumulators per lane, each in the 16-lane order of l2sq16).
 // Rotation of a few queries (nq <= kRotSmallQ): one workgroup of 64 threads per (64 outputs, query),
 // the query staged in LDS (d floats, dynamic), each thread one output's fma chain over the input
-// dims in order with the column loads of M issued 8 ahead. Bit-identical to k_rotate; it spreads a
+// dims in order with the column loads of M issued 64 ahead. Bit-identical to k_rotate; it spreads a
 // single query over d / 64 workgroups instead of leaving each chain behind a dependent load.
 constexpr uint32_t kRotSmallQ = 8;
 __global__ __launch_bounds__(64) void k_rotate_small(const float* __restrict__ x, const float* __restrict__ M,
@@ -277,6 +286,13 @@ __global__ __launch_bounds__(64) void k_rotate_small(const float* __restrict__ x
     const float* col = M + i;
     float acc = 0.0f;
     uint32_t j = 0;
+    for (; j + 64 <= d; j += 64) {
+        float m[64];
+#pragma unroll
+        for (uint32_t u = 0; u < 64; ++u) m[u] = col[(size_t)(j + u) * d];
+#pragma unroll
+        for (uint32_t u = 0; u < 64; ++u) acc = __builtin_fmaf(xq[j + u], m[u], acc);
+    }
     for (; j + 8 <= d; j += 8) {
         float m[8];
 #pragma unroll
@@ -380,7 +396,21 @@ __global__ __launch_bounds__(256) void k_probe_dist(const float* __restrict__ qf
         float acc[QT];
 #pragma unroll
         for (uint32_t r = 0; r < QT; ++r) acc[r] = 0.0f;
-        for (uint32_t j = l; j < d; j += 16) {
+        // lane l takes dims l, l + 16, ... in that order; 8 centroid loads are in flight at once
+        uint32_t j = l;
+        for (; j + 7 * 16 < d; j += 8 * 16) {
+            float xv[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) xv[u] = cr[j + 16 * u];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u)
+#pragma unroll
+                for (uint32_t r = 0; r < QT; ++r) {
+                    float df = qs[r * d + j + 16 * u] - xv[u];
+                    acc[r] = __builtin_fmaf(df, df, acc[r]);
+                }
+        }
+        for (; j < d; j += 16) {
             const float x = cr[j];
 #pragma unroll
             for (uint32_t r = 0; r < QT; ++r) {

@@ -135,17 +135,18 @@ __global__ __launch_bounds__(kListThreads) void k_meso_scatter(MesoArgs a) {
         if (mm[k] != 0xFFFFFFFFu) a.list[a.lstart[mm[k]] + hist[mm[k]] + rank[k]] = val[k];
 }
 
-// Summary offsets of a query's visits: running sum of the visited clusters' sizes.
-__global__ __launch_bounds__(256) void k_meso_soff(MesoArgs a) {
-    __shared__ uint32_t red[256 / kWave + 1];
+// Summary offsets of a query's visits: running sum of the visited clusters' sizes (one wave
+// per query).
+__global__ __launch_bounds__(kWave) void k_meso_soff(MesoArgs a) {
+    __shared__ uint32_t red[2];
     const uint32_t q = blockIdx.x;
     const uint32_t nv = a.nvisit[q];
     uint32_t base = 0;
-    for (uint32_t r0 = 0; r0 < nv; r0 += 256) {
+    for (uint32_t r0 = 0; r0 < nv; r0 += kWave) {
         const uint32_t r = r0 + threadIdx.x;
         const uint32_t x = r < nv ? a.L.n_used[a.order[(size_t)q * a.stride + r]] : 0u;
         uint32_t tot;
-        const uint32_t ex = block_excl_sum<256>(x, red, &tot);
+        const uint32_t ex = block_excl_sum<kWave>(x, red, &tot);
         if (r < nv) a.soff[(size_t)q * a.stride + r] = base + ex;
         base += tot;
     }
