@@ -185,9 +185,28 @@ __device__ __forceinline__ void meso_summarize(const MesoArgs& a, const float* d
         float acc[NQ];
 #pragma unroll
         for (uint32_t u = 0; u < NQ; ++u) acc[u] = 0.0f;
+        // vertical loads in flight per step: 16 for items of one or two queries (a single-query
+        // search), within the registers the 8-query item needs; 8 otherwise
+        constexpr uint32_t VS = NQ <= 2 ? 16 : 8;
         uint32_t j = 0, end = 0;
         for (uint32_t t = 0; t < a.nw; ++t) {
             end = min(end + c_fetch[t], v);
+            if constexpr (VS > 8) {
+                for (; j + VS <= end; j += VS) {
+                    float x[VS];
+#pragma unroll
+                    for (uint32_t s = 0; s < VS; ++s) x[s] = col[(size_t)(j + s) * S];
+#pragma unroll
+                    for (uint32_t u = 0; u < NQ; ++u) {
+                        const_float* pj = qp[u] + j;
+#pragma unroll
+                        for (uint32_t s = 0; s < VS; ++s) {
+                            const float df = pj[s] - x[s];
+                            acc[u] = __builtin_fmaf(df, df, acc[u]);
+                        }
+                    }
+                }
+            }
             for (; j + 8 <= end; j += 8) {
                 float x[8];
 #pragma unroll
