@@ -136,19 +136,30 @@ __global__ __launch_bounds__(kListThreads) void k_meso_scatter(MesoArgs a) {
 }
 
 // Summary offsets of a query's visits: running sum of the visited clusters' sizes (one wave
-// per query).
+// per query; the loads of up to 4 x 64 visits are issued before their in-order scan).
 __global__ __launch_bounds__(kWave) void k_meso_soff(MesoArgs a) {
     __shared__ uint32_t red[2];
     const uint32_t q = blockIdx.x;
     const uint32_t nv = a.nvisit[q];
+    const uint32_t* ord = a.order + (size_t)q * a.stride;
     uint32_t base = 0;
-    for (uint32_t r0 = 0; r0 < nv; r0 += kWave) {
-        const uint32_t r = r0 + threadIdx.x;
-        const uint32_t x = r < nv ? a.L.n_used[a.order[(size_t)q * a.stride + r]] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_sum<kWave>(x, red, &tot);
-        if (r < nv) a.soff[(size_t)q * a.stride + r] = base + ex;
-        base += tot;
+    for (uint32_t r0 = 0; r0 < nv; r0 += 4 * kWave) {
+        uint32_t c[4], x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t r = r0 + u * kWave + threadIdx.x;
+            c[u] = r < nv ? ord[r] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) x[u] = r0 + u * kWave + threadIdx.x < nv ? a.L.n_used[c[u]] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t r = r0 + u * kWave + threadIdx.x;
+            uint32_t tot;
+            const uint32_t ex = block_excl_sum<kWave>(x[u], red, &tot);
+            if (r < nv) a.soff[(size_t)q * a.stride + r] = base + ex;
+            base += tot;
+        }
     }
     if (a.pairs && threadIdx.x == 0) atomicAdd(a.pairs, (unsigned long long)base);
 }
