@@ -147,12 +147,16 @@ def test_every_scan_block_size_gives_the_oracle_results(gpu_lib, kind):
                                                       ("pdx_tree_f32", 3000, 128, 24, 0, 24),
                                                       ("pdx_tree_u8", 2000, 130, 16, 0, 24),
                                                       ("pdx_tree_f32", 20000, 256, 400, 4, 150),
-                                                      ("pdx_tree_u8", 12000, 1024, 300, 3, 70)])
+                                                      ("pdx_tree_u8", 12000, 1024, 300, 3, 70),
+                                                      ("pdx_tree_u8", 24000, 128, 1200, 300, 64)])
 def test_two_phase_meso_scan_matches_pruned_and_oracle(gpu_lib, kind, n, d, nlist, n_meso, nq):
     """The meso level through summaries + replay and through the pruned scan: both equal the oracle.
 
-    The last two cases have meso clusters of 75-100 members (several 64-lane chunks) visited by
-    many queries (several 8-query work items per cluster)."""
+    The fourth and fifth cases have meso clusters of 75-100 members (several 64-lane chunks)
+    visited by many queries (several 8-query work items per cluster). The last one visits 300 meso
+    clusters per query when nprobe >= 600 (more than one 256-visit pass of the summary offsets)
+    and inverts about 15,000 (query, visit) entries (several workgroups of the visit-list
+    kernels)."""
     import torch
 
     path, x, q0 = index_file(kind, n, d, nlist, n_meso=n_meso)
