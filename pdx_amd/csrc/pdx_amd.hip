@@ -606,7 +606,10 @@ void launch_preprocess(const pdxg_index* ix, pdxg_workspace& ws, const float* qu
         src = ws.qnorm.p;
     }
     uint8_t* yq = ix->u8() ? ws.qu8.p : static_cast<uint8_t*>(nullptr);
-    if (!transformed && nq >= kRotTiledMin) {  // large batches: register-blocked tiles
+    // large batches: register-blocked tiles, when they still give every CU two workgroups (at
+    // d = 768 a batch of 1024 makes only 192 tiles, and the 16-query kernel's 768 run faster)
+    const size_t tiles = (size_t)((ix->d + kRotBI - 1) / kRotBI) * ((nq + kRotBQ - 1) / kRotBQ);
+    if (!transformed && nq >= kRotTiledMin && tiles >= 2 * (size_t)ix->num_cus) {
         dim3 g((ix->d + kRotBI - 1) / kRotBI, (nq + kRotBQ - 1) / kRotBQ);
         hipLaunchKernelGGL(k_rotate_tiled, g, dim3(256), 0, st, src, ix->rotation.p, ws.qrot.p, yq, ix->d, nq,
                            ix->qbase, ix->qscale);

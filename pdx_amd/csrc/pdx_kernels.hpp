@@ -309,8 +309,9 @@ __global__ __launch_bounds__(64) void k_rotate_small(const float* __restrict__ x
     }
 }
 
-// Batched rotation (nq >= kRotTiledMin): 64 queries x 64 outputs per workgroup, a 4 x 4 tile per thread
-// (queries ty + 16r, outputs 4tx + c), with a 32-dim chunk of x and of M staged in LDS, so each M
+// Batched rotation (nq >= kRotTiledMin and at least two tiles per CU): 64 queries x 64 outputs
+// per workgroup, a 4 x 4 tile per thread (queries ty + 16r, outputs 4tx + c), with a 32-dim chunk
+// of x and of M staged in LDS, so each M
 // element is read from memory once per 64 queries and each pair of LDS reads feeds 16 fmas. Every
 // output is still one fma chain over the input dims in order (bit-identical to k_rotate).
 constexpr uint32_t kRotBQ = 64, kRotBI = 64, kRotBJ = 32;

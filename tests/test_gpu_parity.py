@@ -78,12 +78,16 @@ def test_preprocess_matches_oracle(gpu_lib, case):
     g = gpu_lib.GpuIndex(pf.load(path))
     rot, qu = g.preprocess(q)
     np.testing.assert_array_equal(rot.view(np.uint32), o.preprocess(q).view(np.uint32))
-    # a batch large enough for the tiled rotation (from 1024 queries, 64 per tile), with a ragged
-    # last tile; its first rows also go through the 16-query kernel and the wide one below
+    # a batch large enough for the tiled rotation (64 queries x 64 outputs per tile, used from 1024
+    # queries when there are two tiles per CU of the MI355X's 256), with a ragged last tile; its
+    # first rows also go through the 16-query kernel (1089 queries make too few tiles at this d)
+    # and the wide one below
     rng = np.random.default_rng(3)
-    qb = np.ascontiguousarray(x[rng.integers(0, n, 1089)] + np.float32(0.01) * rng.standard_normal((1089, d), dtype=np.float32))
+    nb = 64 * -(-512 // ((d + 63) // 64)) + 65
+    qb = np.ascontiguousarray(x[rng.integers(0, n, nb)] + np.float32(0.01) * rng.standard_normal((nb, d), dtype=np.float32))
     rot_b, qu_b = g.preprocess(qb)
     np.testing.assert_array_equal(rot_b.view(np.uint32), o.preprocess(qb).view(np.uint32))
+    np.testing.assert_array_equal(rot_b[:1089].view(np.uint32), g.preprocess(qb[:1089])[0].view(np.uint32))
     np.testing.assert_array_equal(rot_b[:len(q)].view(np.uint32), g.preprocess(qb[:len(q)])[0].view(np.uint32))
     # one and a few queries (the wide small-batch rotation)
     for m in (1, 5, 8):
