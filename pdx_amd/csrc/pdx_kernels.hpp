@@ -390,39 +390,56 @@ __global__ __launch_bounds__(256) void k_probe_dist(const float* __restrict__ qf
     __syncthreads();
     const uint32_t g = threadIdx.x / 16, l = threadIdx.x % 16, G = blockDim.x / 16;
     const uint32_t c0 = blockIdx.y * kProbeCPB;
-    for (uint32_t cc = g; cc < kProbeCPB; cc += G) {
-        uint32_t c = c0 + cc;
+    // a row takes centroids cc and cc + G together, so each query value read from LDS feeds both
+    for (uint32_t cc = g; cc < kProbeCPB; cc += 2 * G) {
+        const uint32_t c = c0 + cc, c2 = c + G;
         if (c >= nc) break;  // uniform across the 16-lane row
+        const bool two = cc + G < kProbeCPB && c2 < nc;  // uniform across the row
         const float* cr = cent + (size_t)c * d;
-        float acc[QT];
+        const float* cr2 = cent + (size_t)(two ? c2 : c) * d;
+        float acc[QT], acc2[QT];
 #pragma unroll
-        for (uint32_t r = 0; r < QT; ++r) acc[r] = 0.0f;
-        // lane l takes dims l, l + 16, ... in that order; 8 centroid loads are in flight at once
+        for (uint32_t r = 0; r < QT; ++r) acc[r] = acc2[r] = 0.0f;
+        // lane l takes dims l, l + 16, ... in that order; 8 loads per centroid are in flight at once
         uint32_t j = l;
         for (; j + 7 * 16 < d; j += 8 * 16) {
-            float xv[8];
+            float xv[8], xw[8];
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) xv[u] = cr[j + 16 * u];
+            for (uint32_t u = 0; u < 8; ++u) {
+                xv[u] = cr[j + 16 * u];
+                xw[u] = cr2[j + 16 * u];
+            }
 #pragma unroll
             for (uint32_t u = 0; u < 8; ++u)
 #pragma unroll
                 for (uint32_t r = 0; r < QT; ++r) {
-                    float df = qs[r * d + j + 16 * u] - xv[u];
+                    const float qv = qs[r * d + j + 16 * u];
+                    const float df = qv - xv[u], dg = qv - xw[u];
                     acc[r] = __builtin_fmaf(df, df, acc[r]);
+                    acc2[r] = __builtin_fmaf(dg, dg, acc2[r]);
                 }
         }
         for (; j < d; j += 16) {
-            const float x = cr[j];
+            const float x = cr[j], y = cr2[j];
 #pragma unroll
             for (uint32_t r = 0; r < QT; ++r) {
-                float df = qs[r * d + j] - x;
+                const float qv = qs[r * d + j];
+                const float df = qv - x, dg = qv - y;
                 acc[r] = __builtin_fmaf(df, df, acc[r]);
+                acc2[r] = __builtin_fmaf(dg, dg, acc2[r]);
             }
         }
 #pragma unroll
         for (uint32_t r = 0; r < QT; ++r) {
             float s = tree16(acc[r]);
             if (l == 0 && r < nqt) out[(size_t)(q0 + r) * nc + c] = s;
+        }
+        if (two) {
+#pragma unroll
+            for (uint32_t r = 0; r < QT; ++r) {
+                float s = tree16(acc2[r]);
+                if (l == 0 && r < nqt) out[(size_t)(q0 + r) * nc + c2] = s;
+            }
         }
     }
 }
