@@ -330,17 +330,39 @@ __global__ __launch_bounds__(256) void k_rotate_tiled(const float* __restrict__ 
     for (uint32_t r = 0; r < 4; ++r)
 #pragma unroll
         for (uint32_t c = 0; c < 4; ++c) acc[r][c] = 0.0f;
-    for (uint32_t j0 = 0; j0 < d; j0 += kRotBJ) {
-        for (uint32_t t = threadIdx.x; t < kRotBQ * kRotBJ; t += 256) {
+    // the next chunk's x and M values are loaded into registers while the current one is used
+    constexpr uint32_t kPX = kRotBQ * kRotBJ / 256, kPM = kRotBJ * kRotBI / 256;
+    float px[kPX], pm[kPM];
+    auto fetch = [&](uint32_t j0) {
+#pragma unroll
+        for (uint32_t u = 0; u < kPX; ++u) {
+            const uint32_t t = threadIdx.x + 256 * u;
             const uint32_t qq = t / kRotBJ, jj = t % kRotBJ, q = q0 + qq, j = j0 + jj;
-            xs[qq][jj] = (q < nq && j < d) ? x[(size_t)q * d + j] : 0.0f;
+            px[u] = (q < nq && j < d) ? x[(size_t)q * d + j] : 0.0f;
         }
-        for (uint32_t t = threadIdx.x; t < kRotBJ * kRotBI; t += 256) {
+#pragma unroll
+        for (uint32_t u = 0; u < kPM; ++u) {
+            const uint32_t t = threadIdx.x + 256 * u;
             const uint32_t jj = t / kRotBI, ii = t % kRotBI, j = j0 + jj, i = i0 + ii;
-            ms[jj][ii] = (j < d && i < d) ? M[(size_t)j * d + i] : 0.0f;
+            pm[u] = (j < d && i < d) ? M[(size_t)j * d + i] : 0.0f;
+        }
+    };
+    fetch(0);
+    for (uint32_t j0 = 0; j0 < d; j0 += kRotBJ) {
+#pragma unroll
+        for (uint32_t u = 0; u < kPX; ++u) {
+            const uint32_t t = threadIdx.x + 256 * u;
+            xs[t / kRotBJ][t % kRotBJ] = px[u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPM; ++u) {
+            const uint32_t t = threadIdx.x + 256 * u;
+            ms[t / kRotBI][t % kRotBI] = pm[u];
         }
         __syncthreads();
+        if (j0 + kRotBJ < d) fetch(j0 + kRotBJ);
         const uint32_t jl = min(kRotBJ, d - j0);  // only the real dims enter the chains
+#pragma unroll 8
         for (uint32_t jj = 0; jj < jl; ++jj) {
             const float4 m = *reinterpret_cast<const float4*>(&ms[jj][4 * tx]);
             float xv[4];
