@@ -77,7 +77,10 @@ pdxg_status pdxg_index_destroy(pdxg_index* index);
 
 /* Re-upload one cluster after Append/Delete/compaction on the host
  * (Cluster::AppendEmbedding / DeleteEmbedding / CompactCluster, cluster.hpp:121-154,260-294).
- * level: 1 = data clusters, 0 = meso clusters. max_capacity must be unchanged. */
+ * level: 1 = data clusters, 0 = meso clusters. max_capacity must be unchanged. The call waits for
+ * searches still being planned on other threads (pdxg_search_device, servers) and for every kernel
+ * in flight on the device before it writes; searches launched after it see the new cluster.
+ * Captured graphs keep the sizes of their capture and must be re-created. */
 pdxg_status pdxg_index_update_cluster(pdxg_index* index, uint32_t level, uint32_t cluster_id,
                                       const pdxg_cluster_desc* cluster);
 
@@ -298,10 +301,12 @@ pdxg_status pdxg_dense_create(pdxg_index* index, uint32_t precision, uint64_t ex
 pdxg_status pdxg_dense_destroy(pdxg_dense* dense);
 pdxg_status pdxg_dense_search(pdxg_dense* dense, const pdxg_dense_search_args* args, void* stream);
 /* Appends n rows (host buffers: n * d floats, n ids); raw rows are normalised and rotated with the
- * index's rotation first. An id that is already live is replaced. Synchronous. */
+ * index's rotation first. An id that is already live is replaced. Synchronous; waits first for every
+ * kernel in flight on the device (a search of this copy on any stream). */
 pdxg_status pdxg_dense_append(pdxg_dense* dense, const float* rows, const uint32_t* ids, uint32_t n,
                               uint32_t rows_transformed);
-/* Deletes rows by id (host buffer); ids without a live row are ignored. Synchronous. */
+/* Deletes rows by id (host buffer); ids without a live row are ignored. Synchronous; waits first
+ * for every kernel in flight on the device. */
 pdxg_status pdxg_dense_delete(pdxg_dense* dense, const uint32_t* ids, uint32_t n);
 /* Device timing of the searches of this copy (HIP events on their stream): when enabled,
  * pdxg_dense_stage_times waits for the last search and writes the milliseconds of [0] query

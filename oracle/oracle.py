@@ -48,7 +48,9 @@ def lib():
     if _lib is not None:
         return _lib
     name = "liboracle_v4.so" if _has_avx512() else "liboracle_v3.so"
-    path = os.path.join(_BUILD, name)
+    # PDX_ORACLE_LIB names another build of pdx_oracle.cpp (used to check that the pinning
+    # tests fail on a deliberately altered copy of the oracle)
+    path = os.environ.get("PDX_ORACLE_LIB") or os.path.join(_BUILD, name)
     if not os.path.exists(path):
         raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
     L = ctypes.CDLL(path)
@@ -75,6 +77,9 @@ def lib():
     L.orc_filtered_search.restype = ctypes.c_int
     L.orc_filtered_search.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_int, ctypes.c_uint32,
                                       ctypes.c_uint32, _u8p, _u32p, _f32p, _u32p, _u32p]
+    L.orc_search_meso.restype = ctypes.c_int
+    L.orc_search_meso.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                  _u32p, _f32p, _u32p]
     L.orc_create.restype = ctypes.c_void_p
     L.orc_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                              _f32p, ctypes.c_uint32, ctypes.c_void_p, _f32p, ctypes.c_uint32, ctypes.c_void_p, _f32p]
@@ -223,6 +228,19 @@ class OracleIndex:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return ids, dists, idists, counts, nbytes
+
+    def search_meso(self, queries_transformed: np.ndarray, k: int, nprobe: int):
+        """Meso-level search of a two-level index (the cluster choice of PDXTreeIndex::Append,
+        index.hpp:596-607). Returns (ids [nq,k] L1 cluster ids, dists, counts)."""
+        q = np.ascontiguousarray(queries_transformed, np.float32).reshape(-1, self.d)
+        nq = q.shape[0]
+        ids = np.zeros((nq, k), np.uint32)
+        dists = np.zeros((nq, k), np.float32)
+        counts = np.zeros(nq, np.uint32)
+        if lib().orc_search_meso(self._h, _p(q, _f32p), nq, k, nprobe, _p(ids, _u32p), _p(dists, _f32p),
+                                 _p(counts, _u32p)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return ids, dists, counts
 
     def cluster_offsets(self) -> np.ndarray:
         out = np.zeros(self.num_clusters + 1, np.uint64)

@@ -819,6 +819,32 @@ int orc_search(void* h, const float* queries, uint32_t nq, int transformed, cons
     return 0;
 }
 
+// Meso-level search alone (two-level indexes): the top-level PDXearch<F32> over the L1 centroids,
+// as PDXTreeIndex::Append runs it to pick a cluster (index.hpp:596-607: nprobe over the meso
+// clusters, k results, already transformed vector). Row stride of the outputs is k.
+int orc_search_meso(void* h, const float* queries_transformed, uint32_t nq, uint32_t k, uint32_t nprobe,
+                    uint32_t* out_ids, float* out_dists, uint32_t* out_counts) {
+    try {
+        auto* idx = static_cast<orc::Index*>(h);
+        if (!idx->is_tree()) throw std::runtime_error("not a two-level index");
+        const uint32_t d = idx->l1.d;
+        for (uint32_t q = 0; q < nq; ++q) {
+            orc::Stats st;
+            auto res = orc::search_level<false>(idx->l0, idx->ratios, queries_transformed + (size_t)q * d, k, nprobe,
+                                                nullptr, nullptr, &st);
+            out_counts[q] = (uint32_t)res.size();
+            for (size_t i = 0; i < res.size(); ++i) {
+                out_ids[(size_t)q * k + i] = res[i].id;
+                out_dists[(size_t)q * k + i] = res[i].dist;
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 // Filtered search of one level-1 probe (PDXIndex::FilteredSearch, index.hpp:273-280).
 // `slot_sel`: selection flag per (cluster offset + slot), total_capacity entries.
 int orc_filtered_search(void* h, const float* query, int transformed, uint32_t k, uint32_t nprobe,

@@ -292,6 +292,19 @@ class GpuIndex:
                                     _ptr(qu, ctypes.POINTER(ctypes.c_uint8)) if qu is not None else None))
         return rot, qu
 
+    def search_meso(self, queries_transformed: np.ndarray, k: int, nprobe: int):
+        """Meso-level search of a two-level index (pdxg_search_meso): (ids [nq,k], dists, counts)."""
+        q = np.ascontiguousarray(queries_transformed, np.float32).reshape(-1, self.data.d)
+        nq = q.shape[0]
+        ids = np.zeros((nq, k), np.uint32)
+        dists = np.zeros((nq, k), np.float32)
+        counts = np.zeros(nq, np.uint32)
+        check(lib().pdxg_search_meso(self._h, _ptr(q, ctypes.POINTER(ctypes.c_float)), nq, k, nprobe,
+                                     _ptr(ids, ctypes.POINTER(ctypes.c_uint32)),
+                                     _ptr(dists, ctypes.POINTER(ctypes.c_float)),
+                                     _ptr(counts, ctypes.POINTER(ctypes.c_uint32))))
+        return ids, dists, counts
+
     def filtered_search(self, query: np.ndarray, k: int, nprobe: int, selection: np.ndarray,
                         transformed: bool = False):
         q = np.ascontiguousarray(query, np.float32).reshape(self.data.d)

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -271,6 +272,7 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block, bool summ) 
 
 struct pdxg_workspace {
     int device = 0;
+    const pdxg_index* index = nullptr;  // the index the workspace was planned for
     bool timing = false;
     DevBuf<unsigned long long> stage_bytes;  // [meso scan, data scan] of the last search when timing
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -319,6 +321,9 @@ struct pdxg_index {
     int max_lds = 65536;
     int num_cus = 1;
     std::mutex mu;  // serialises the host-buffer API (shared workspace)
+    // searches plan and launch under a shared lock; pdxg_index_update_cluster takes it exclusively
+    // and waits for the device before it rewrites cluster data, counts or tombstones
+    std::shared_mutex rw;
     std::unique_ptr<pdxg_workspace> ws;
     DevBuf<float> h_queries, h_dists;
     DevBuf<uint32_t> h_k, h_np, h_ids, h_idists, h_counts;
@@ -520,6 +525,13 @@ void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax
 
 // Phase 1 of the two-phase meso scan: visit lists inverted per meso cluster, then the
 // partial-distance summaries of every visited (query, meso member) pair.
+// The two-phase meso scan runs when the workspace holds its buffers and the plan asks for it, the
+// meso level has no tombstones (the replay does not mask) and the meso clusters a query may visit
+// still fit the per-query summary region planned at workspace creation.
+bool summary_applies(const pdxg_index* ix, const pdxg_workspace& ws) {
+    return ws.summ_ok && ws.summ_use && ix->l0.tomb_total == 0 && ix->l0.top_prefix[ws.nv0_max] <= ws.region;
+}
+
 void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, hipStream_t st) {
     const Level& L = ix->l0;
     MesoArgs m{};
@@ -627,6 +639,7 @@ void launch_preprocess(const pdxg_index* ix, pdxg_workspace& ws, const float* qu
 
 // The full search sequence on device buffers.
 void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a, hipStream_t st) {
+    std::shared_lock<std::shared_mutex> reading(ix->rw);
     const uint32_t nq = a.num_queries;
     require(nq <= ws.max_queries, "more queries than the workspace holds");
     require(a.k_max >= 1 && a.k_max <= ws.max_k, "k_max must be in [1, workspace max_k]");
@@ -735,8 +748,7 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s0.out_stride = ws.k0_max;
         s0.stage_bytes = sb;
         s0.queue = queues[0];
-        const bool summ = ws.summ_ok && ws.summ_use && ix->l0.tomb_total == 0 &&
-                          ix->l0.top_prefix[ws.nv0_max] <= ws.region;
+        const bool summ = summary_applies(ix, ws);
         ws.last_summ = summ;
         if (summ) {
             launch_meso_summary(ix, ws, nq, st);
@@ -787,6 +799,7 @@ pdxg_workspace* host_workspace(pdxg_index* ix, uint32_t nq, uint32_t max_k, uint
                                                                        : std::max(max_nprobe, ws ? ws->max_nprobe : 0u);
         ws.reset(new pdxg_workspace);
         ws->device = ix->device;
+        ws->index = ix;
         plan_workspace(ix, mq, mk, mp, *ws);
         alloc_workspace(ix, *ws);
     }
@@ -956,10 +969,13 @@ pdxg_status pdxg_index_update_cluster(pdxg_index* ix, uint32_t level, uint32_t c
         require(level == 1 || ix->tree(), "flat indexes have no level 0");
         DeviceGuard g(ix->device);
         std::lock_guard<std::mutex> lk(ix->mu);
+        std::unique_lock<std::shared_mutex> writing(ix->rw);
         Level& L = level == 1 ? ix->l1 : ix->l0;
         require(cluster_id < L.nc, "cluster id out of range");
         require(cd->max_capacity == L.cap[cluster_id], "max_capacity change requires re-creating the index");
         L.check_cluster(*cd);
+        // searches already launched (on any stream, e.g. a server's slots) read this cluster
+        HIP_CHECK(hipDeviceSynchronize());
         L.upload_cluster(cluster_id, *cd);
         L.sync_counts();
         HIP_CHECK(hipDeviceSynchronize());
@@ -980,6 +996,7 @@ pdxg_status pdxg_workspace_create(pdxg_index* ix, uint32_t max_queries, uint32_t
         DeviceGuard g(ix->device);
         std::unique_ptr<pdxg_workspace> ws(new pdxg_workspace);
         ws->device = ix->device;
+        ws->index = ix;
         plan_workspace(ix, max_queries, max_k, max_nprobe, *ws);
         alloc_workspace(ix, *ws);
         *out = ws.release();
@@ -1081,7 +1098,9 @@ pdxg_status pdxg_workspace_set_order(pdxg_workspace* ws, int mode) {
 
 int pdxg_workspace_order(const pdxg_workspace* ws) { return ws ? static_cast<int>(ws->order_mode) : -1; }
 
-int pdxg_workspace_meso_summary(const pdxg_workspace* ws) { return ws && ws->summ_ok && ws->summ_use ? 1 : 0; }
+int pdxg_workspace_meso_summary(const pdxg_workspace* ws) {
+    return ws && ws->index && summary_applies(ws->index, *ws) ? 1 : 0;
+}
 
 pdxg_status pdxg_workspace_get_plan(pdxg_workspace* ws, uint32_t* out2) {
     return guarded([&] {
