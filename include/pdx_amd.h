@@ -175,6 +175,18 @@ int pdxg_workspace_order(const pdxg_workspace* ws);
 pdxg_status pdxg_search_device(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,
                                void* stream);
 
+/* pdxg_search_device without allocation. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments (device buffers) and results as pdxg_search_device; its per-call
+ * scratch is carved out of one fixed 64 MiB array that the library keeps on each device it runs on
+ * until the process ends, so no workspace is needed and nothing is allocated or freed. The meso
+ * level of two-level indexes always runs the pruned scan (the two-phase scan's summaries need far
+ * more room). A call whose scratch would exceed 64 MiB (about 2 x num_queries x d x 4 bytes plus
+ * num_queries x (centroids probed + nprobe) x 8) returns PDXG_ERR_UNSUPPORTED. All calls on one GPU
+ * share its array: calls that overlap in time on one GPU (other threads, other streams) race and
+ * must be serialised by the caller. Asynchronous on `stream`. */
+pdxg_status pdxg_search_device_noalloc(pdxg_index* index, const pdxg_search_args* args, void* stream);
+
 /* Query preprocessing only (normalise if the index is normalised, rotate; searcher.hpp:601-613),
  * host buffers: out_rotated num_queries*d floats, out_quantized (U8 indexes, may be NULL)
  * num_queries*d bytes (ScalarQuantizer::QuantizeEmbedding, quantizers/scalar.hpp:76-94). */

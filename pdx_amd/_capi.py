@@ -84,7 +84,7 @@ PDXG_DENSE_BF16, PDXG_DENSE_FP8, PDXG_DENSE_NO_RESIDUAL = 0, 1, 0x100
 # every symbol include/pdx_amd.h declares
 EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_index_device_bytes",
            "pdxg_search", "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_workspace_bytes",
-           "pdxg_search_device", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
+           "pdxg_search_device", "pdxg_search_device_noalloc", "pdxg_preprocess", "pdxg_filtered_search", "pdxg_last_error",
            "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing", "pdxg_workspace_stage_times",
            "pdxg_workspace_stage_bytes", "pdxg_workspace_meso_times", "pdxg_workspace_meso_pairs",
            "pdxg_workspace_set_plan", "pdxg_workspace_get_plan", "pdxg_workspace_autotune",
@@ -120,6 +120,7 @@ def lib():
     L.pdxg_workspace_bytes.restype = ctypes.c_size_t
     L.pdxg_workspace_bytes.argtypes = [vp, u32, u32, u32]
     L.pdxg_search_device.argtypes = [vp, vp, ctypes.POINTER(SearchArgs), vp]
+    L.pdxg_search_device_noalloc.argtypes = [vp, ctypes.POINTER(SearchArgs), vp]
     L.pdxg_preprocess.argtypes = [vp, f32p, u32, u32, f32p, u8p]
     L.pdxg_filtered_search.argtypes = [vp, f32p, u32, u32, u32, u8p, u32p, f32p, u32p, u32p]
     L.pdxg_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
@@ -171,7 +172,8 @@ def lib():
                  "pdxg_dense_delete", "pdxg_dense_info", "pdxg_dense_enable_timing", "pdxg_dense_stage_times"):
         getattr(L, name).restype = ctypes.c_int
     for name in ("pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster", "pdxg_search",
-                 "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device", "pdxg_preprocess",
+                 "pdxg_workspace_create", "pdxg_workspace_destroy", "pdxg_search_device",
+                 "pdxg_search_device_noalloc", "pdxg_preprocess",
                  "pdxg_filtered_search", "pdxg_device_info", "pdxg_search_meso", "pdxg_workspace_enable_timing",
                  "pdxg_workspace_stage_times", "pdxg_workspace_stage_bytes", "pdxg_workspace_set_plan",
                  "pdxg_workspace_get_plan", "pdxg_workspace_autotune", "pdxg_server_create", "pdxg_server_destroy", "pdxg_server_submit",
@@ -321,6 +323,18 @@ class GpuIndex:
                                          _ptr(cnt, ctypes.POINTER(ctypes.c_uint32))))
         n = int(cnt[0])
         return ids[:n], dists[:n], idists[:n]
+
+
+def search_device_noalloc(index: GpuIndex, queries_ptr: int, nq: int, k_ptr: int, nprobe_ptr: int, k_max: int,
+                          nprobe_max: int, out_stride: int, out_ids: int, out_dists: int, out_idists: int,
+                          out_counts: int, out_bytes: int = 0, transformed: bool = False, stream: int = 0,
+                          k_all: int = 0, nprobe_all: int = 0) -> None:
+    """pdxg_search_device_noalloc: the search of Workspace.search with no workspace and no device
+    allocation (device pointers as ints, 0 = NULL; asynchronous on `stream`)."""
+    a = SearchArgs(nq, queries_ptr or None, int(transformed), k_ptr or None, nprobe_ptr or None, k_all,
+                   nprobe_all, k_max, nprobe_max, out_stride, out_ids or None, out_dists or None,
+                   out_idists or None, out_counts or None, out_bytes or None)
+    check(lib().pdxg_search_device_noalloc(index.handle, ctypes.byref(a), ctypes.c_void_p(stream or 0)))
 
 
 class Workspace:

@@ -11,6 +11,7 @@
 #include <shared_mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pdx_amd.h"
@@ -71,20 +72,28 @@ template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;  // false: a view into memory the buffer does not own (never freed)
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
     }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
         HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
         n = count;
+    }
+    void view(T* ptr, size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        owned = false;
     }
     void ensure(size_t count) {
         if (count > n || !p) alloc(std::max(count, n * 5 / 4));
@@ -1123,6 +1132,93 @@ pdxg_status pdxg_search_device(pdxg_index* ix, pdxg_workspace* ws, const pdxg_se
         require(ix && ws && args, "null argument");
         DeviceGuard g(ix->device);
         search_device(ix, *ws, *args, static_cast<hipStream_t>(stream));
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// Search without allocation: the per-call workspace of pdxg_search_device carved out of one
+// fixed device array. Each device holds its own copy of `scratch` (loaded with the code object)
+// until the process ends; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device; the device must be current.
+unsigned char* scratch_on(int device) {
+    require(device >= 0 && device < kMaxScratchDevices, "device ordinal out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            throw Error(PDXG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    return static_cast<unsigned char*>(g_scratch_ptr[device]);
+}
+
+// Points the workspace's buffers (as alloc_workspace sizes them, without the two-phase meso
+// summaries) into [base, base + cap); returns the bytes the layout needs (views past `cap` are
+// never used: the caller rejects the call first).
+size_t carve_workspace(const pdxg_index* ix, pdxg_workspace& ws, bool need_qnorm, unsigned char* base) {
+    size_t off = 0;
+    auto take = [&](auto& buf, size_t count) {
+        using T = typename std::remove_reference<decltype(*buf.p)>::type;
+        off = (off + 255) & ~size_t(255);
+        count = std::max<size_t>(count, 1);
+        buf.view(reinterpret_cast<T*>(base + off), count);
+        off += count * sizeof(T);
+    };
+    const size_t nq = ws.max_queries, d = ix->d;
+    const size_t nc_probe = ix->tree() ? ix->l0.nc : ix->l1.nc;
+    if (need_qnorm) take(ws.qnorm, nq * d);
+    take(ws.qrot, nq * d);
+    take(ws.probe, nq * nc_probe);
+    if (ix->u8()) take(ws.qu8, nq * d);
+    take(ws.k1, nq);
+    take(ws.nv1, nq);
+    take(ws.k0, nq);
+    take(ws.nv0, nq);
+    take(ws.perm, nq);
+    take(ws.queue, 2);
+    take(ws.order, nq * ws.order_stride);
+    if (ix->tree()) {
+        take(ws.ids0, nq * ws.k0_max);
+        take(ws.cnt0, nq);
+    }
+    return off;
+}
+
+}  // namespace
+
+pdxg_status pdxg_search_device_noalloc(pdxg_index* ix, const pdxg_search_args* args, void* stream) {
+    return guarded([&] {
+        require(ix && args, "null argument");
+        DeviceGuard g(ix->device);
+        unsigned char* base = scratch_on(ix->device);
+        const pdxg_search_args& a = *args;
+        require(a.k_max >= 1, "k_max must be >= 1");
+        pdxg_workspace ws;  // host-side description only: every buffer is a view into `scratch`
+        ws.device = ix->device;
+        ws.index = ix;
+        plan_workspace(ix, std::max(a.num_queries, 1u), a.k_max, a.nprobe_max, ws);
+        ws.summ_ok = false;  // the meso level runs the pruned scan (its summaries need far more room)
+        const bool need_qnorm = ix->normalized && !a.queries_transformed;
+        const size_t need = carve_workspace(ix, ws, need_qnorm, base);
+        require(need <= kScratchBytes,
+                "this search needs " + std::to_string(need) + " bytes of scratch; pdxg_search_device_noalloc holds " +
+                    std::to_string(kScratchBytes) + " (use fewer queries per call, or pdxg_search_device)",
+                PDXG_ERR_UNSUPPORTED);
+        // every buffer is written before it is read within the call (the work-queue counters by a
+        // fill kernel), so the previous call's contents never leak into this one
+        search_device(ix, ws, a, static_cast<hipStream_t>(stream));
     });
 }
 
