@@ -44,7 +44,7 @@ CONFIGS = {
     "c2": dict(n=1_000_000, d=768, centers=1024, sigma=0.15, unit=True, kind="pdx_f32", nlist=1000, n_meso=0,
                normalize=False, nq=10_000, batch=1024, k=10, nprobe=32, nprobe_range=None, gt=1000),
     "c3": dict(n=5_000_000, d=1024, centers=4096, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
-               normalize=True, nq=8192, batch=4096, k=20, nprobe=0, nprobe_range=(8, 128), gt=512),
+               normalize=True, nq=8192, batch=4096, k=20, nprobe=0, nprobe_range=(8, 128), gt=512, f32_twin=True),
     # serving over a resident two-level SQ8 index; each query draws k and nprobe from the sets
     "c4": dict(n=10_000_000, d=1536, centers=8192, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
                normalize=True, nq=8192, batch=4096, k=100, k_choices=(10, 20, 100), nprobe=0, nprobe_range=None,
@@ -141,7 +141,10 @@ def main():
     ap.add_argument("--precisions", default="", help="dense configurations: comma-separated subset of the precisions")
     ap.add_argument("--sigma-per-coord", action="store_true", help="unit-norm mixtures: sigma per coordinate")
     ap.add_argument("--target-recall", type=float, default=0.95)
-    ap.add_argument("--parity-queries", type=int, default=256)
+    ap.add_argument("--parity-queries", type=int, default=-1,
+                    help="queries checked against the oracle (-1: one whole timed batch)")
+    ap.add_argument("--no-f32-twin", action="store_true",
+                    help="skip the F32 two-level leg over the same clustering (the recall-gated number)")
     ap.add_argument("--index-cache", default="",
                     help="index file: loaded when present, else written after the build (e.g. to profile without "
                          "the build's kernels)")
@@ -248,14 +251,18 @@ def main():
     def recall_now():
         ids = np.zeros((gtq, K), np.uint32)
         cnt = np.zeros(gtq, np.uint32)
+        idist = np.zeros((gtq, K), np.uint32)
         for s in range(0, gtq, B):
             m = min(B, gtq - s)
             run(s, m)
             torch.cuda.synchronize()
             ids[s:s + m] = out_ids[:m].cpu().numpy().view(np.uint32)
             cnt[s:s + m] = out_c[:m].cpu().numpy()
+            idist[s:s + m] = out_i[:m].cpu().numpy().view(np.uint32)
+        state["gt_idist"] = idist
         k10 = min(10, K)
         state["recall10"] = recall(ids[:, :k10], np.minimum(cnt, k10), gt, k10)
+        state["gt_ids"], state["gt_cnt"] = ids, cnt
         if per_k:  # recall@k with each query's own k
             return float(np.mean([len(set(ids[i, :cnt[i]].tolist()) & set(gt[i, :k_q[i]].tolist())) / k_q[i]
                                   for i in range(gtq)]))
@@ -303,9 +310,9 @@ def main():
         + (f"; two-phase meso scan: {state['ws'].meso_summary}" if tree else ""))
 
     parity = None
-    if args.parity_queries:
-        parity = check_parity(idx, q, nprobe_q, k_q, min(args.parity_queries, B), run, out_ids, out_d, out_i,
-                              out_c, torch)
+    n_par = B if args.parity_queries < 0 else min(args.parity_queries, B)
+    if n_par:
+        parity = check_parity(idx, q, nprobe_q, k_q, n_par, run, out_ids, out_d, out_i, out_c, torch)
 
     # timed leg: `steps` batches, inputs resident in HBM, one batch in flight at a time
     nb = max(1, NQ // B)
@@ -465,9 +472,20 @@ def main():
         serving["by_slots"] = {sp_: {"saturated_qps": l["saturated_qps"], "p99_ms_at_50%": l["load"]["50%"]["p99_ms"]}
                                for sp_, l in zip(specs, legs)}
     cpu = None if args.no_cpu else cpu_baseline(idx, q, nprobe_q, k_q, args)
+    decomposition = None
+    if u8 and not per_k:
+        decomposition = sq8_exhaustive(idx, g, q[:gtq], gt, state["gt_idist"], state["gt_cnt"], K, torch, dev)
+    twin = None
+    if cfg.get("f32_twin") and not args.no_f32_twin:
+        twin = f32_twin_leg(args, cfg, idx, x, q, gt, nprobe_q, torch, dev, _capi, builder, sp)
+    gate_met = bool(state["recall10"] >= args.target_recall)
 
     line = {
-        "metric": "QPS at recall@10>=0.95 (batched GPU search; plain per-query GPU path and p50/p99 reported beside it)",
+        "metric": ("QPS at recall@10>=%.2f (batched GPU search; plain per-query GPU path and p50/p99 reported beside "
+                   "it)" % args.target_recall) if gate_met else
+                  ("QPS of the batched GPU search at the configuration's own nprobe; its recall@10 is %.4f, below the "
+                   "%.2f gate (SQ8 rounding caps it on this data, see recall_decomposition); the gated figure is "
+                   "f32_twin (the F32 two-level path over the same clustering)" % (state["recall10"], args.target_recall)),
         "value": round(qps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -486,7 +504,10 @@ def main():
                    "scan_order": {"batch": ws.order, "batch64": ws64.order, "one_query": ws1.order}},
         ("recall_at_k_per_query" if per_k else f"recall_at_{K}"): round(rec, 4),
         "recall_at_10": round(state["recall10"], 4),
+        "recall_gate_met": gate_met,
         "recall_sweep": sweep,
+        "recall_decomposition": decomposition,
+        "f32_twin": twin,
         "recall_at_4x_max_nprobe": ceiling,
         "parity_vs_oracle": parity,
         "plain_per_query_qps": round(plain_qps, 1),
@@ -740,6 +761,157 @@ def cpu_baseline_dense(idx, q, sel, K, args):
     log(f"cpu oracle ({orc.isa()}): exhaustive filtered search {qps:.2f} QPS over {n1} queries on 1 thread")
     return {"value": round(qps, 3), "unit": "queries/s", "cores": 1, "kind": "port", "isa": orc.isa(),
             "sample": f"{n1} pre-rotated queries, filtered search over every list (nprobe = all), 1 thread"}
+
+
+def _overlap(a, b):
+    """Size of the multiset intersection of two integer lists."""
+    from collections import Counter
+
+    return sum((Counter(a.tolist()) & Counter(b.tolist())).values())
+
+
+def sq8_exhaustive(idx, g, q, gt, s_idist, s_cnt, K, torch, dev, chunk=1 << 19):
+    """Separates the two things that can cap recall on an SQ8 index: an exhaustive search over every
+    stored code with exact integer distances (no probe, no pruning), compared with the FP64 ground
+    truth, shows what SQ8 rounding allows; the pruned GPU search's integer distances compared with
+    that exhaustive top-k (as multisets, so ties do not matter) show what the probe and ADSampling
+    pruning lose."""
+    from pdx_amd import pdxfile as pf
+
+    t0 = time.time()
+    L = idx.l1
+    codes = np.concatenate([pf.from_pdx(c.data, c.n_used, c.cap, L.d, L.v, L.h, True) for c in L.clusters])
+    rows = np.concatenate([c.ids[:c.n_used] for c in L.clusters]).astype(np.int64)
+    _, qu = g.preprocess(q)                     # the search's own query codes
+    qt = torch.from_numpy(qu.astype(np.float64)).to(dev)
+    qn = (qt * qt).sum(1)
+    best_d = torch.full((len(q), K), float("inf"), dtype=torch.float64, device=dev)
+    best_i = torch.zeros((len(q), K), dtype=torch.int64, device=dev)
+    for s in range(0, len(codes), chunk):
+        xt = torch.from_numpy(codes[s:s + chunk]).to(dev).to(torch.float64)
+        dd = (xt * xt).sum(1)[None, :] - 2.0 * (qt @ xt.T) + qn[:, None]   # exact: integers below 2^53
+        vd, vi = torch.topk(dd, min(K, dd.shape[1]), dim=1, largest=False)
+        cd = torch.cat([best_d, vd], 1)
+        ci = torch.cat([best_i, vi + s], 1)
+        o = torch.argsort(cd, dim=1)[:, :K]
+        best_d, best_i = torch.gather(cd, 1, o), torch.gather(ci, 1, o)
+    ex_d = best_d.cpu().numpy().round().astype(np.int64)
+    ex_ids = rows[best_i.cpu().numpy()]
+    del codes
+    torch.cuda.empty_cache()
+    k10 = min(10, K)
+    ex_rec = recall(ex_ids[:, :k10].astype(np.uint32), np.full(len(q), k10), gt, k10)
+    same = np.mean([_overlap(s_idist[i, :min(k10, s_cnt[i])].astype(np.int64), ex_d[i, :k10]) / k10
+                    for i in range(len(q))])
+    same_k = np.mean([_overlap(s_idist[i, :s_cnt[i]].astype(np.int64), ex_d[i, :K]) / K for i in range(len(q))])
+    log(f"SQ8 exhaustive exact-integer search: recall@10 {ex_rec:.4f} vs FP64; the pruned search returns "
+        f"{same:.4f} of its top-10 integer distances ({same_k:.4f} of top-{K}) ({time.time() - t0:.1f}s)")
+    return {"queries": len(q), "sq8_exhaustive_recall_at_10": round(float(ex_rec), 4),
+            "pruned_search_top10_distances_in_exhaustive_sq8_top10": round(float(same), 4),
+            f"pruned_search_top{K}_distances_in_exhaustive_sq8_top{K}": round(float(same_k), 4)}
+
+
+def f32_twin_leg(args, cfg, idx, x, q, gt, nprobe_q, torch, dev, _capi, builder, sp):
+    """IndexPDXIVFTree (F32) over the clustering of the SQ8 index (same lists, meso level, order of
+    members and row ids; rows stored as the normalised + rotated floats): recall@10, nprobe scaled
+    up from the configuration's draws until recall@10 reaches the target, QPS of the timed batches,
+    the data-level scan's HBM roofline and parity with the oracle on one whole batch."""
+    from oracle.oracle import OracleIndex
+
+    t0 = time.time()
+    tw = builder.float_twin(idx, x, device=0)
+    g = _capi.GpuIndex(tw)
+    log(f"f32 twin built in {time.time() - t0:.1f}s, resident {g.device_bytes / 2**30:.2f} GiB")
+    B, K, NQ = cfg["batch"], cfg["k"], len(q)
+    gtq = len(gt)
+    nlist = tw.l1.nc
+    qd = torch.from_numpy(q).to(dev)
+    ids = torch.zeros((B, K), dtype=torch.int32, device=dev)
+    dd = torch.zeros((B, K), dtype=torch.float32, device=dev)
+    cnt = torch.zeros(B, dtype=torch.int32, device=dev)
+    st = {}
+
+    def make(arr):
+        st["npd"] = torch.from_numpy(arr.astype(np.int32)).to(dev)
+        st["npm"] = int(arr.max())
+        st["ws"] = _capi.Workspace(g, B, K, st["npm"])
+        st["ws"].enable_timing(True)
+
+    def run(s, m):
+        st["ws"].search(qd[s:s + m].data_ptr(), m, 0, st["npd"][s:s + m].data_ptr(), K, st["npm"], K, ids.data_ptr(),
+                        dd.data_ptr(), 0, cnt.data_ptr(), 0, stream=sp, k_all=K)
+
+    def rec10():
+        out = np.zeros((gtq, K), np.uint32)
+        c = np.zeros(gtq, np.uint32)
+        for s in range(0, gtq, B):
+            m = min(B, gtq - s)
+            run(s, m)
+            torch.cuda.synchronize()
+            out[s:s + m] = ids[:m].cpu().numpy().view(np.uint32)
+            c[s:s + m] = cnt[:m].cpu().numpy()
+        k10 = min(10, K)
+        return recall(out[:, :k10], np.minimum(c, k10), gt, k10)
+
+    arr = nprobe_q.copy()
+    scale = 1
+    while True:
+        make(arr)
+        r10 = rec10()
+        log(f"f32 twin: recall@10 {r10:.4f} at nprobe x{scale}")
+        if r10 >= args.target_recall or int(arr.min()) >= nlist:
+            break
+        arr = np.minimum(nlist, arr * 2).astype(np.uint32)
+        scale *= 2
+    plan = st["ws"].autotune(qd[:B].data_ptr(), B, 0, st["npd"][:B].data_ptr(), K, st["npm"], K, ids.data_ptr(),
+                             dd.data_ptr(), 0, cnt.data_ptr(), stream=sp, k_all=K)
+    o = OracleIndex(data=tw)
+    oi, od, _, oc, _ = o.search(q[:B], K, arr[:B], threads=min(16, os.cpu_count() or 1))
+    del o
+    run(0, B)
+    torch.cuda.synchronize()
+    gi = ids.cpu().numpy().view(np.uint32)
+    gd = dd.cpu().numpy()
+    gc = cnt.cpu().numpy().view(np.uint32)
+    par_ok = bool(np.array_equal(gc, oc)) and all(
+        np.array_equal(gi[r, :oc[r]], oi[r, :oc[r]]) and np.array_equal(gd[r, :oc[r]].view(np.uint32),
+                                                                         od[r, :oc[r]].view(np.uint32))
+        for r in range(B))
+    nb = max(1, NQ // B)
+    for i in range(args.warmup):
+        run((i % nb) * B, B)
+    torch.cuda.synchronize()
+    stage = np.zeros(4)
+    sbytes = np.zeros(2)
+    t = time.perf_counter()
+    for i in range(args.steps):
+        run((i % nb) * B, B)
+        stage += st["ws"].stage_times()
+        sbytes += st["ws"].stage_bytes()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    qps = B * args.steps / el
+    stage /= args.steps
+    sbytes /= args.steps
+    gbs = sbytes[1] / (stage[3] * 1e-3) / 1e9
+    log(f"f32 twin: {qps:.0f} QPS at recall@10 {r10:.4f} (nprobe x{scale}); stage ms {stage.round(4)}; data scan "
+        f"{gbs:.0f} GB/s; parity vs oracle on {B} queries: {par_ok}")
+    out = {"path": "IndexPDXIVFTree (F32) over the SQ8 index's clustering", "qps": round(qps, 1),
+           "ms_per_step": round(el * 1e3 / args.steps, 4), "recall_at_10": round(float(r10), 4),
+           "nprobe": "U[%d,%d] x %d (capped at %d lists)" % (cfg["nprobe_range"] + (scale, nlist))
+           if cfg.get("nprobe_range") else str(int(arr[0])),
+           "recall_gate_met": bool(r10 >= args.target_recall), "scan_block": list(plan),
+           "meso_two_phase": st["ws"].meso_summary,
+           "stage_ms": {"preprocess": round(float(stage[0]), 4), "probe": round(float(stage[1]), 4),
+                        "meso_scan": round(float(stage[2]), 4), "scan": round(float(stage[3]), 4)},
+           "data_scan": {"bound": "hbm", "kernel": "k_scan<F32> (data-level pruned PDX scan)",
+                         "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": int(sbytes[1])},
+           "parity_vs_oracle": {"queries": B, "ids_and_float_distances_equal": par_ok},
+           "resident_gib": round(g.device_bytes / 2**30, 2)}
+    del g, tw
+    torch.cuda.empty_cache()
+    return out
 
 
 def check_parity(idx, q, nprobe_q, k_q, n, run, out_ids, out_d, out_i, out_c, torch):

@@ -282,3 +282,23 @@ def build_index(data: np.ndarray, index_type: str = "pdx_tree_u8", *, distance_m
         M.clusters = _clusters_from_assignment(cent, a0, n0, M, np.arange(nlist))
         idx.l0 = M
     return idx
+
+
+def float_twin(index: pf.IndexData, data: np.ndarray, device=None) -> pf.IndexData:
+    """The F32 counterpart of an SQ8 index with the same clustering: every cluster keeps its members
+    (in slot order), row ids and capacity, centroids and the meso level are shared, and the rows are
+    stored as the normalised + rotated float32 vectors instead of their SQ8 codes. `data` holds the
+    raw rows by row id."""
+    twin = {pf.TYPE_U8: pf.TYPE_F32, pf.TYPE_TREE_U8: pf.TYPE_TREE_F32}
+    if index.type not in twin:
+        raise ValueError("float_twin needs an SQ8 index")
+    L = index.l1
+    be = _Backend(device)
+    y, _, _ = be.preprocess(np.ascontiguousarray(data, np.float32), index.matrix, L.normalized)
+    F = pf.Level(L.d, L.v, L.h, False, L.normalized, [], L.centroids)
+    for c in L.clusters:
+        if c.tombstones:
+            raise ValueError("float_twin needs clusters without tombstones")
+        ids = c.ids[:c.n_used]
+        F.clusters.append(pf.make_cluster(y[ids.astype(np.int64)], ids, F, cap=c.cap))
+    return pf.IndexData(type=twin[index.type], matrix=index.matrix, l1=F, l0=index.l0)

@@ -117,22 +117,44 @@ def _regions(d: int, v: int, h: int, u8: bool):
     return regs
 
 
+def _blocks(buf: np.ndarray, cap: int, v: int, h: int, u8: bool):
+    """Views of a strided PDX buffer: (vertical blocks [groups, cap, width], vertical tail
+    [cap, r] or None, horizontal blocks [h/64, cap, 64]). F32 vertical dims are groups of width 1,
+    U8 ones groups of 4 with a tail of v % 4 dims."""
+    w = U8_GROUP if u8 else 1
+    g = v // w
+    vert = buf[:g * w * cap].reshape(g, cap, w)
+    r = v - g * w
+    tail = buf[g * w * cap: v * cap].reshape(cap, r) if r else None
+    hor = buf[v * cap: (v + h) * cap].reshape(h // H_DIM, cap, H_DIM)
+    return vert, tail, hor
+
+
 def to_pdx(vecs: np.ndarray, cap: int, v: int, h: int, u8: bool) -> np.ndarray:
     """Row-major vectors (n x d) -> strided PDX buffer of cap * d elements."""
     n, d = vecs.shape
     assert n <= cap
     buf = np.zeros(cap * d, np.uint8 if u8 else np.float32)
-    for start, width in _regions(d, v, h, u8):
-        reg = buf[start * cap: (start + width) * cap].reshape(cap, width)
-        reg[:n] = vecs[:, start:start + width]
+    vert, tail, hor = _blocks(buf, cap, v, h, u8)
+    w = vert.shape[2]
+    g = vert.shape[0]
+    vert[:, :n, :] = vecs[:, :g * w].reshape(n, g, w).transpose(1, 0, 2)
+    if tail is not None:
+        tail[:n] = vecs[:, g * w:v]
+    hor[:, :n, :] = vecs[:, v:v + h].reshape(n, h // H_DIM, H_DIM).transpose(1, 0, 2)
     return buf
 
 
 def from_pdx(buf: np.ndarray, n: int, cap: int, d: int, v: int, h: int, u8: bool) -> np.ndarray:
     """Strided PDX buffer -> row-major vectors of the first n slots."""
     out = np.zeros((n, d), buf.dtype)
-    for start, width in _regions(d, v, h, u8):
-        out[:, start:start + width] = buf[start * cap: (start + width) * cap].reshape(cap, width)[:n]
+    vert, tail, hor = _blocks(buf, cap, v, h, u8)
+    w = vert.shape[2]
+    g = vert.shape[0]
+    out[:, :g * w] = vert[:, :n, :].transpose(1, 0, 2).reshape(n, g * w)
+    if tail is not None:
+        out[:, g * w:v] = tail[:n]
+    out[:, v:v + h] = hor[:, :n, :].transpose(1, 0, 2).reshape(n, h)
     return out
 
 
