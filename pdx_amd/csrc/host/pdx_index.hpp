@@ -135,29 +135,33 @@ struct Cluster {  // cluster.hpp:17-427
           min_capacity(static_cast<uint32_t>(n * MIN_CAPACITY_THRESHOLD)), num_dimensions(d), indices(max_cap, 0),
           data((size_t)max_cap * d, data_t(0)) {}
 
-    // Element ranges of the PDX layout: (first dim, width) in storage order (layout.hpp:21-87).
-    std::vector<std::pair<uint32_t, uint32_t>> Regions() const {
-        auto s = GetPDXDimensionSplit(num_dimensions);
-        std::vector<std::pair<uint32_t, uint32_t>> r;
+    // Calls f(first dim, width) for each element range of the PDX layout in storage order
+    // (layout.hpp:21-87): F32 vertical dims one by one, U8 vertical dims in groups of 4 (and a
+    // tail of v % 4), then the 64-dim horizontal blocks.
+    template <class F>
+    void ForEachRegion(F&& f) const {
+        const auto s = GetPDXDimensionSplit(num_dimensions);
+        const uint32_t v = s.vertical_dimensions;
         if (Q == F32) {
-            for (uint32_t j = 0; j < s.vertical_dimensions; ++j) r.push_back({j, 1});
+            for (uint32_t j = 0; j < v; ++j) f(j, 1u);
         } else {
             uint32_t j = 0;
-            for (; j + U8_INTERLEAVE_SIZE <= s.vertical_dimensions; j += U8_INTERLEAVE_SIZE) r.push_back({j, U8_INTERLEAVE_SIZE});
-            if (j < s.vertical_dimensions) r.push_back({j, s.vertical_dimensions - j});
+            for (; j + U8_INTERLEAVE_SIZE <= v; j += U8_INTERLEAVE_SIZE) f(j, U8_INTERLEAVE_SIZE);
+            if (j < v) f(j, v - j);
         }
-        for (uint32_t h = 0; h < s.horizontal_dimensions; h += H_DIM_SIZE) r.push_back({s.vertical_dimensions + h, H_DIM_SIZE});
-        return r;
+        for (uint32_t h = 0; h < s.horizontal_dimensions; h += H_DIM_SIZE) f(v + h, H_DIM_SIZE);
     }
 
     void InsertEmbedding(uint32_t slot, uint32_t row_id, const data_t* emb) {  // cluster.hpp:341-384
-        for (auto [start, width] : Regions())
+        ForEachRegion([&](uint32_t start, uint32_t width) {
             std::memcpy(&data[(size_t)start * max_capacity + (size_t)slot * width], emb + start, width * sizeof(data_t));
+        });
         indices[slot] = row_id;
     }
     void ReadEmbedding(uint32_t slot, data_t* out) const {  // cluster.hpp:299-337
-        for (auto [start, width] : Regions())
+        ForEachRegion([&](uint32_t start, uint32_t width) {
             std::memcpy(out + start, &data[(size_t)start * max_capacity + (size_t)slot * width], width * sizeof(data_t));
+        });
     }
     void CopySlot(uint32_t src, uint32_t dst) {
         std::vector<data_t> tmp(num_dimensions);
@@ -211,17 +215,18 @@ struct Cluster {  // cluster.hpp:17-427
         return sizeof(*this) + num_embeddings * sizeof(uint32_t) + (size_t)num_embeddings * num_dimensions * sizeof(data_t);
     }
     void SaveCompact(std::ostream& out) const {  // cluster.hpp:185-221
-        for (auto [start, width] : Regions())
+        ForEachRegion([&](uint32_t start, uint32_t width) {
             out.write(reinterpret_cast<const char*>(&data[(size_t)start * max_capacity]),
                       (std::streamsize)((size_t)width * num_embeddings * sizeof(data_t)));
+        });
     }
     void LoadCompact(const char*& p, const char* end) {  // cluster.hpp:225-255
-        for (auto [start, width] : Regions()) {
+        ForEachRegion([&](uint32_t start, uint32_t width) {
             size_t bytes = (size_t)width * num_embeddings * sizeof(data_t);
             if (p + bytes > end) throw std::runtime_error("index file truncated");
             std::memcpy(&data[(size_t)start * max_capacity], p, bytes);
             p += bytes;
-        }
+        });
     }
 };
 
@@ -443,6 +448,29 @@ inline void quantize(const float* x, float base, float scale, uint8_t* out, uint
     }
 }
 
+// ScalarQuantizer::DequantizeEmbedding (quantizers/scalar.hpp:96-106)
+inline void dequantize(const uint8_t* q, float base, float scale, float* out, uint32_t d) {
+    for (uint32_t i = 0; i < d; ++i) out[i] = static_cast<float>(q[i]) / scale + base;
+}
+
+// F32 L2 over d dims in the 16-lane order of the reference's Horizontal kernel
+// (distance_computers/avx512_computers.hpp:47-79): lane l accumulates dims l, l+16, ... with fma,
+// lanes then combine (i, i+8), (i, i+4), (i, i+1), (i, i+2).
+inline float l2sq16(const float* a, const float* b, uint32_t d) {
+    float acc[16] = {};
+    for (uint32_t base = 0; base < d; base += 16) {
+        const uint32_t lim = std::min<uint32_t>(16, d - base);
+        for (uint32_t l = 0; l < lim; ++l) {
+            const float df = a[base + l] - b[base + l];
+            acc[l] = std::fmaf(df, df, acc[l]);
+        }
+    }
+    float x[8], r[4];
+    for (int i = 0; i < 8; ++i) x[i] = acc[i] + acc[i + 8];
+    for (int i = 0; i < 4; ++i) r[i] = x[i] + x[i + 4];
+    return (r[0] + r[1]) + (r[2] + r[3]);
+}
+
 }  // namespace detail
 
 // Shared machinery of the flat and two-level indexes.
@@ -635,16 +663,19 @@ class PDXIndexBase : public IPDXIndex {
     // index.hpp:213-239 / 483-514
     void Save(const std::string& path) override {
         RequireBuilt();
-        bool moved = false;
+        // Compaction changes a cluster whenever it holds tombstones, also when they all sit at its
+        // tail and no row moves (used_capacity shrinks): the device copy is then re-created.
+        bool changed = false;
         for (uint32_t c = 0; c < index.num_clusters; ++c) {
-            for (auto [row, slot] : index.clusters[c].CompactCluster()) {
-                row_id_cluster_mapping[row] = {c, slot};
-                moved = true;
-            }
+            changed |= !index.clusters[c].tombstones.empty();
+            for (auto [row, slot] : index.clusters[c].CompactCluster()) row_id_cluster_mapping[row] = {c, slot};
         }
         if constexpr (TREE)
-            for (auto& c : index.l0.clusters) moved |= !c.CompactCluster().empty();
-        if (moved) device.reset();
+            for (auto& c : index.l0.clusters) {
+                changed |= !c.tombstones.empty();
+                c.CompactCluster();
+            }
+        if (changed) device.reset();
         std::ofstream out(path, std::ios::binary);
         if (!out) throw std::runtime_error("cannot write " + path);
         uint8_t t = static_cast<uint8_t>(IndexType());
@@ -711,11 +742,16 @@ class PDXIndexBase : public IPDXIndex {
             index.l0.centroids.resize((size_t)index.l0.num_clusters * d);
             std::memcpy(index.l0.centroids.data(), p, cb);
             p += cb;
+            // The file holds no L1 centroids (ivf_wrapper.hpp:314-380); each one is the vector its
+            // meso cluster stores for that L1 cluster, so they are read back from there (the
+            // maintenance's splits and merges and the data level's own probe need them).
+            index.centroids.assign((size_t)n1 * d, 0.0f);
             for (uint32_t mc = 0; mc < index.l0.num_clusters; ++mc) {
                 auto& c = index.l0.clusters[mc];
                 for (uint32_t s = 0; s < c.num_embeddings; ++s) {
                     if (c.indices[s] >= n1) throw std::runtime_error("meso-cluster member is not an L1 cluster");
                     index.clusters[c.indices[s]].mesocluster_id = mc;
+                    c.ReadEmbedding(s, &index.centroids[(size_t)c.indices[s] * d]);
                 }
             }
             index.l0.ComputeClusterOffsets();
@@ -824,6 +860,7 @@ class PDXTreeIndex : public PDXIndexBase<Q, true> {  // index.hpp:449-1729
     using Base::index;
     using Base::row_id_cluster_mapping;
     using Base::DELETED_MARKER;
+    using data_t = typename Base::data_t;
 
   public:
     using Base::Base;
@@ -853,7 +890,9 @@ class PDXTreeIndex : public PDXIndexBase<Q, true> {  // index.hpp:449-1729
         else slot = cl.AppendEmbedding((uint32_t)row_id, rot.data());
         row_id_cluster_mapping.emplace_back(cid, slot);
         index.total_num_embeddings++;
+        restructured_ = false;
         CheckClusterHealth(cid);
+        SyncDevice(cid);
     }
 
     // index.hpp:619-638
@@ -867,42 +906,444 @@ class PDXTreeIndex : public PDXIndexBase<Q, true> {  // index.hpp:449-1729
         index.clusters[cid].DeleteEmbedding(slot);
         row_id_cluster_mapping[row_id] = {DELETED_MARKER, DELETED_MARKER};
         index.total_num_embeddings--;
+        restructured_ = false;
         CheckClusterHealth(cid);
+        SyncDevice(cid);
     }
 
   private:
-    // index.hpp:1241-1255. A full cluster is compacted when it holds tombstones; a full cluster
-    // without tombstones gets a larger buffer (the reference splits it with k-means, which this
-    // build does not do). Under-full clusters are left in place (the reference merges them).
-    void CheckClusterHealth(uint32_t cid) {
-        auto& cl = index.clusters[cid];
-        if (cl.used_capacity == cl.max_capacity) {
-            if (cl.num_embeddings < cl.used_capacity) {
-                for (auto [row, s] : cl.CompactCluster()) row_id_cluster_mapping[row] = {cid, s};
-            } else {
-                Grow(cl);
-                index.ComputeClusterOffsets();
-                device.reset();  // capacities changed: re-upload on next use
-                return;
-            }
+    using dist_t = typename std::conditional<Q == U8, uint32_t, float>::type;
+    static constexpr size_t kMaxSplitNeighbors = 32;  // index.hpp:948 (max_neighbors)
+    static constexpr uint32_t kSplitIters = 4;        // index.hpp:1036-1046, 1330-1342
+
+    bool restructured_ = false;  // a split or merge renumbered clusters or changed capacities
+    uint64_t l1_merges_ = 0;     // L1 merges so far (each renumbers the last cluster)
+
+    // A cluster changed in place is patched on the device; after a split or merge the device copy
+    // is re-created at the next search (cluster count, capacities and meso membership changed).
+    void SyncDevice(uint32_t cid) {
+        if (restructured_) {
+            device.reset();
+            restructured_ = false;
+        } else {
+            device.template update<Q>(1, cid, index.clusters[cid]);
         }
-        device.template update<Q>(1, cid, cl);
     }
 
-    void Grow(Cluster<Q>& cl) {
-        uint32_t ncap = std::max(static_cast<uint32_t>(cl.max_capacity * Cluster<Q>::CAPACITY_THRESHOLD), cl.max_capacity + 1);
-        Cluster<Q> g(cl.num_embeddings, ncap, cl.num_dimensions);
-        g.used_capacity = cl.used_capacity;
-        g.min_capacity = cl.min_capacity;
-        g.id = cl.id;
-        g.mesocluster_id = cl.mesocluster_id;
-        std::vector<typename Cluster<Q>::data_t> tmp(cl.num_dimensions);
-        for (uint32_t s = 0; s < cl.used_capacity; ++s) {
-            cl.ReadEmbedding(s, tmp.data());
-            g.InsertEmbedding(s, cl.indices[s], tmp.data());
+    bool NormalizedCentroids() const {
+        return config.normalize || DistanceMetricRequiresNormalization(config.distance_metric);
+    }
+    const float* Centroid(uint32_t cid) const { return &index.centroids[(size_t)cid * index.num_dimensions]; }
+
+    // Row-major copies of the first n slots (the cluster has no tombstones below n).
+    std::vector<data_t> RowMajor(const Cluster<Q>& c, uint32_t n) const {
+        std::vector<data_t> out((size_t)n * c.num_dimensions);
+        for (uint32_t s = 0; s < n; ++s) c.ReadEmbedding(s, &out[(size_t)s * c.num_dimensions]);
+        return out;
+    }
+    // index.hpp:861-881
+    std::vector<float> Dequantized(const data_t* raw, size_t n) const {
+        const uint32_t d = index.num_dimensions;
+        std::vector<float> out(n * d);
+        if constexpr (Q == U8) {
+            for (size_t i = 0; i < n; ++i)
+                detail::dequantize(raw + i * d, index.quantization_base, index.quantization_scale, &out[i * d], d);
+        } else if (n) {
+            std::memcpy(out.data(), raw, n * d * sizeof(float));
         }
-        g.tombstones = cl.tombstones;
-        cl = std::move(g);
+        return out;
+    }
+    // index.hpp:884-895
+    uint32_t QuantizeAndAppend(uint32_t cid, uint32_t row_id, const float* emb) {
+        auto& c = index.clusters[cid];
+        if constexpr (Q == U8) {
+            std::vector<uint8_t> qv(index.num_dimensions);
+            detail::quantize(emb, index.quantization_base, index.quantization_scale, qv.data(), index.num_dimensions);
+            return c.AppendEmbedding(row_id, qv.data());
+        } else {
+            return c.AppendEmbedding(row_id, emb);
+        }
+    }
+    // index.hpp:920-942: mean of the accumulated sum (the k-means centroid when the group is
+    // empty), normalised when the index is
+    std::vector<float> CentroidMean(const std::vector<float>& sum, size_t count, const float* fallback) const {
+        const uint32_t d = index.num_dimensions;
+        std::vector<float> out(d);
+        if (count == 0) {
+            std::memcpy(out.data(), fallback, d * sizeof(float));
+        } else {
+            const float inv = 1.0f / static_cast<float>(count);
+            for (uint32_t j = 0; j < d; ++j) out[j] = sum[j] * inv;
+        }
+        if (NormalizedCentroids()) host::normalize_row(out.data(), out.data(), d);
+        return out;
+    }
+    // index.hpp:945-981: the other L1 clusters of the meso cluster, at most the 32 nearest
+    std::vector<uint32_t> NeighborClusterIds(uint32_t cid, uint32_t meso, const float* centroid) const {
+        std::vector<uint32_t> ids;
+        const auto& m = index.l0.clusters[meso];
+        for (uint32_t p = 0; p < m.used_capacity; ++p)
+            if (!m.tombstones.count(p) && m.indices[p] != cid) ids.push_back(m.indices[p]);
+        if (ids.size() > kMaxSplitNeighbors) {
+            std::vector<std::pair<float, uint32_t>> nd;
+            for (uint32_t n : ids) nd.push_back({detail::l2sq16(centroid, Centroid(n), index.num_dimensions), n});
+            std::nth_element(nd.begin(), nd.begin() + kMaxSplitNeighbors, nd.end(),
+                             [](const auto& a, const auto& b) { return a.first < b.first; });
+            ids.clear();
+            for (size_t i = 0; i < kMaxSplitNeighbors; ++i) ids.push_back(nd[i].second);
+        }
+        return ids;
+    }
+    // index.hpp:983-996
+    uint32_t PositionInMeso(uint32_t cid, uint32_t meso) const {
+        const auto& m = index.l0.clusters[meso];
+        for (uint32_t p = 0; p < m.used_capacity; ++p)
+            if (!m.tombstones.count(p) && m.indices[p] == cid) return p;
+        throw std::runtime_error("FindPositionInMesoCluster: L1 cluster " + std::to_string(cid) +
+                                 " not found in L0 meso-cluster " + std::to_string(meso));
+    }
+    uint32_t LiveMesoMembers(uint32_t meso) const { return index.l0.clusters[meso].num_embeddings; }
+    // Index into `cand` of the nearest candidate (L2, ties to the first) for each of n rows
+    // (the reference's batch_computer::FindNearestNeighbor).
+    std::vector<uint32_t> Nearest(const float* embs, size_t n, const std::vector<const float*>& cand) const {
+        const uint32_t d = index.num_dimensions;
+        std::vector<uint32_t> out(n, 0);
+        for (size_t i = 0; i < n; ++i) {
+            float best = std::numeric_limits<float>::infinity();
+            for (uint32_t c = 0; c < cand.size(); ++c) {
+                const float v = detail::l2sq16(embs + i * d, cand[c], d);
+                if (v < best) {
+                    best = v;
+                    out[i] = c;
+                }
+            }
+        }
+        return out;
+    }
+    // index.hpp:1691-1727: the distance of every used slot of a cluster to one vector, vertical
+    // dims then the 64-dim horizontal blocks (exact integers for U8).
+    std::vector<dist_t> DistancesToCluster(const data_t* query, const Cluster<Q>& c) const {
+        const uint32_t d = index.num_dimensions, v = index.num_vertical_dimensions;
+        std::vector<dist_t> out(c.used_capacity);
+        std::vector<data_t> row(d);
+        for (uint32_t s = 0; s < c.used_capacity; ++s) {
+            c.ReadEmbedding(s, row.data());
+            if constexpr (Q == U8) {
+                uint32_t acc = 0;
+                for (uint32_t j = 0; j < d; ++j) {
+                    const int df = (int)query[j] - (int)row[j];
+                    acc += (uint32_t)(df * df);
+                }
+                out[s] = acc;
+            } else {
+                float acc = 0.0f;
+                for (uint32_t j = 0; j < v; ++j) {
+                    const float df = query[j] - row[j];
+                    acc = std::fmaf(df, df, acc);
+                }
+                for (uint32_t hb = v; hb < d; hb += H_DIM_SIZE) acc += detail::l2sq16(query + hb, row.data() + hb, H_DIM_SIZE);
+                out[s] = acc;
+            }
+        }
+        return out;
+    }
+    // A new L1 cluster holding `count` row-major embeddings (StoreClusterEmbeddings).
+    Cluster<Q> NewCluster(uint32_t id, uint32_t meso, const std::vector<data_t>& embs, const std::vector<uint32_t>& ids) const {
+        const uint32_t d = index.num_dimensions;
+        Cluster<Q> c(static_cast<uint32_t>(ids.size()), d);
+        c.id = id;
+        c.mesocluster_id = meso;
+        for (size_t i = 0; i < ids.size(); ++i) c.InsertEmbedding((uint32_t)i, ids[i], &embs[i * d]);
+        return c;
+    }
+
+    // index.hpp:1241-1255
+    void CheckClusterHealth(uint32_t cid, bool allow_merges = true) {
+        auto& cl = index.clusters[cid];
+        if (cl.used_capacity == cl.max_capacity) {
+            if (cl.num_embeddings < cl.used_capacity) {  // compacting is cheaper than splitting
+                for (auto [row, s] : cl.CompactCluster()) row_id_cluster_mapping[row] = {cid, s};
+            } else {
+                SplitCluster(cid);
+            }
+        } else if (allow_merges && cl.num_embeddings <= cl.min_capacity) {
+            DestroyAndMergeCluster(cid);
+        }
+    }
+
+    // index.hpp:1257-1311. A cluster that is the only live member of its meso cluster, or the only
+    // cluster, is kept: its rows would have nowhere to go (the reference has no guard there).
+    void DestroyAndMergeCluster(uint32_t cid) {
+        const uint32_t d = index.num_dimensions;
+        const uint32_t meso = index.clusters[cid].mesocluster_id;
+        if (index.num_clusters <= 1 || LiveMesoMembers(meso) <= 1) return;
+        restructured_ = true;
+        l1_merges_++;
+        auto& cl = index.clusters[cid];
+        cl.CompactCluster();
+        const uint32_t n = cl.num_embeddings;
+        std::vector<uint32_t> rows(cl.indices.begin(), cl.indices.begin() + n);
+        std::vector<data_t> raw = RowMajor(cl, n);
+        std::vector<float> embs = Dequantized(raw.data(), n);
+        index.l0.clusters[meso].DeleteEmbedding(PositionInMeso(cid, meso));
+        index.l0.total_num_embeddings--;
+        const uint32_t last = index.num_clusters - 1;
+        if (cid != last) {  // swap-and-pop
+            index.clusters[cid] = std::move(index.clusters[last]);
+            auto& mv = index.clusters[cid];
+            mv.id = cid;
+            std::memcpy(&index.centroids[(size_t)cid * d], Centroid(last), d * sizeof(float));
+            for (uint32_t s = 0; s < mv.used_capacity; ++s)
+                if (!mv.tombstones.count(s)) row_id_cluster_mapping[mv.indices[s]] = {cid, s};
+            auto& mm = index.l0.clusters[mv.mesocluster_id];
+            mm.indices[PositionInMeso(last, mv.mesocluster_id)] = cid;
+        }
+        index.clusters.pop_back();
+        index.centroids.resize((size_t)last * d);
+        index.num_clusters--;
+        index.ComputeClusterOffsets();
+        index.l0.ComputeClusterOffsets();
+        ReassignEmbeddings(rows.data(), embs.data(), n, meso, false);
+        CheckL0ClusterHealth(meso);
+    }
+
+    // index.hpp:1314-1611 (the cluster is full and has no tombstones). A 2-means split of its
+    // members; a member closer to another centroid of the meso cluster than to both halves is
+    // reassigned; members of the (up to 32) neighbouring clusters that are not closer to their own
+    // centroid than to both halves move to the nearer half.
+    void SplitCluster(uint32_t cid) {
+        restructured_ = true;
+        const uint32_t d = index.num_dimensions;
+        const uint32_t meso = index.clusters[cid].mesocluster_id;
+        const uint32_t n = index.clusters[cid].num_embeddings;
+        std::vector<data_t> raw = RowMajor(index.clusters[cid], n);
+        std::vector<float> embs = Dequantized(raw.data(), n);
+        std::vector<float> old_c(Centroid(cid), Centroid(cid) + d);
+        std::vector<uint32_t> neighbors = NeighborClusterIds(cid, meso, old_c.data());
+        // ComputeKMeans(..., 2 clusters, sampling 1.0, 4 iterations, 1 thread). The reference asks
+        // for angular (spherical) k-means at every split; on vectors that are not normalised its
+        // unit-norm centroids would lie far from the data and leave both halves empty, so angular
+        // k-means is used when the index's vectors are normalised and plain Lloyd otherwise.
+        auto km = host::kmeans(embs.data(), n, d, 2, kSplitIters, config.seed, 1.0f, NormalizedCentroids(), 1);
+        const float* ca = km.centroids.data();
+        const float* cb = ca + d;
+        std::vector<uint32_t> ga, gb, rest;
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* e = &embs[(size_t)i * d];
+            const float dold = detail::l2sq16(e, old_c.data(), d);
+            const float da = detail::l2sq16(e, ca, d), db = detail::l2sq16(e, cb, d);
+            const float mn = std::min(da, db);
+            bool elsewhere = false;
+            if (mn > dold)
+                for (uint32_t c : neighbors)
+                    if (detail::l2sq16(e, Centroid(c), d) < mn) {
+                        elsewhere = true;
+                        break;
+                    }
+            if (elsewhere) rest.push_back(i);
+            else (da <= db ? ga : gb).push_back(i);
+        }
+        std::vector<data_t> ea, eb;
+        std::vector<uint32_t> ia, ib;
+        std::vector<float> sa(d, 0.0f), sb(d, 0.0f);
+        auto gather = [&](const std::vector<uint32_t>& g, std::vector<data_t>& eo, std::vector<uint32_t>& io,
+                          std::vector<float>& sum) {
+            for (uint32_t i : g) {
+                eo.insert(eo.end(), &raw[(size_t)i * d], &raw[(size_t)(i + 1) * d]);
+                io.push_back(index.clusters[cid].indices[i]);
+                for (uint32_t j = 0; j < d; ++j) sum[j] += embs[(size_t)i * d + j];
+            }
+        };
+        gather(ga, ea, ia, sa);
+        gather(gb, eb, ib, sb);
+        std::vector<float> rest_embs;
+        std::vector<uint32_t> rest_ids;
+        for (uint32_t i : rest) {
+            rest_embs.insert(rest_embs.end(), &embs[(size_t)i * d], &embs[(size_t)(i + 1) * d]);
+            rest_ids.push_back(index.clusters[cid].indices[i]);
+        }
+        // members of the neighbours that are nearer to a half than to their own centroid move
+        std::vector<data_t> qa(d), qb(d), qo(d);
+        if constexpr (Q == U8) {
+            detail::quantize(ca, index.quantization_base, index.quantization_scale, qa.data(), d);
+            detail::quantize(cb, index.quantization_base, index.quantization_scale, qb.data(), d);
+        } else {
+            std::memcpy(qa.data(), ca, d * sizeof(float));
+            std::memcpy(qb.data(), cb, d * sizeof(float));
+        }
+        std::vector<data_t> row(d);
+        std::vector<float> rowf(d);
+        for (uint32_t nb : neighbors) {
+            auto& nc = index.clusters[nb];
+            if constexpr (Q == U8)
+                detail::quantize(Centroid(nb), index.quantization_base, index.quantization_scale, qo.data(), d);
+            else
+                std::memcpy(qo.data(), Centroid(nb), d * sizeof(float));
+            auto down = DistancesToCluster(qo.data(), nc);
+            auto dta = DistancesToCluster(qa.data(), nc);
+            auto dtb = DistancesToCluster(qb.data(), nc);
+            for (uint32_t p = 0; p < nc.used_capacity; ++p) {
+                if (nc.tombstones.count(p)) continue;
+                if (down[p] < dta[p] && down[p] < dtb[p]) continue;
+                nc.ReadEmbedding(p, row.data());
+                if constexpr (Q == U8) detail::dequantize(row.data(), index.quantization_base, index.quantization_scale, rowf.data(), d);
+                else std::memcpy(rowf.data(), row.data(), d * sizeof(float));
+                const bool to_a = dta[p] <= dtb[p];
+                const uint32_t rid = nc.indices[p];
+                nc.DeleteEmbedding(p);
+                (to_a ? ea : eb).insert((to_a ? ea : eb).end(), row.begin(), row.end());
+                (to_a ? ia : ib).push_back(rid);
+                auto& sum = to_a ? sa : sb;
+                for (uint32_t j = 0; j < d; ++j) sum[j] += rowf[j];
+            }
+        }
+        std::vector<float> ta = CentroidMean(sa, ia.size(), ca), tb = CentroidMean(sb, ib.size(), cb);
+        const uint32_t bid = index.num_clusters;
+        index.clusters[cid] = NewCluster(cid, meso, ea, ia);
+        index.clusters.push_back(NewCluster(bid, meso, eb, ib));
+        index.num_clusters++;
+        std::memcpy(&index.centroids[(size_t)cid * d], ta.data(), d * sizeof(float));
+        index.centroids.insert(index.centroids.end(), tb.begin(), tb.end());
+        for (size_t i = 0; i < ia.size(); ++i) row_id_cluster_mapping[ia[i]] = {cid, (uint32_t)i};
+        for (size_t i = 0; i < ib.size(); ++i) row_id_cluster_mapping[ib[i]] = {bid, (uint32_t)i};
+        // the meso cluster: the old centroid out, both halves in. It may split in between; it is not
+        // merged there (a merge would renumber the meso cluster the second half goes to), only by
+        // the check at the end.
+        {
+            auto& m = index.l0.clusters[meso];
+            m.DeleteEmbedding(PositionInMeso(cid, meso));
+            m.CompactCluster();
+            m.AppendEmbedding(cid, ta.data());
+        }
+        CheckL0ClusterHealth(meso, false);
+        index.l0.clusters[meso].AppendEmbedding(bid, tb.data());
+        index.l0.total_num_embeddings++;
+        if (!rest_ids.empty()) ReassignEmbeddings(rest_ids.data(), rest_embs.data(), (uint32_t)rest_ids.size(), meso);
+        index.ComputeClusterOffsets();
+        index.l0.ComputeClusterOffsets();
+        CheckL0ClusterHealth(meso);
+    }
+
+    // index.hpp:1616-1685: each row to the nearest live L1 cluster of the meso cluster.
+    void ReassignEmbeddings(const uint32_t* rows, const float* embs, uint32_t n, uint32_t meso, bool allow_merges = true) {
+        const uint32_t d = index.num_dimensions;
+        auto candidates = [&] {
+            std::vector<uint32_t> cand;
+            const auto& m = index.l0.clusters[meso];
+            for (uint32_t p = 0; p < m.used_capacity; ++p)
+                if (!m.tombstones.count(p)) cand.push_back(m.indices[p]);
+            if (cand.empty()) throw std::runtime_error("ReassignEmbeddings: meso-cluster has no live L1 cluster");
+            return cand;
+        };
+        std::vector<uint32_t> cand;
+        std::vector<uint32_t> a;
+        uint64_t merges_seen = ~0ull;
+        for (uint32_t i = 0; i < n; ++i) {
+            // nearest candidates for the remaining rows; recomputed only when a merge triggered by
+            // an earlier row renumbered clusters (splits keep every existing cluster id)
+            if (merges_seen != l1_merges_) {
+                cand = candidates();
+                std::vector<const float*> cp;
+                for (uint32_t c : cand) cp.push_back(Centroid(c));
+                a = Nearest(embs + (size_t)i * d, n - i, cp);
+                a.insert(a.begin(), i, 0u);
+                merges_seen = l1_merges_;
+            }
+            const uint32_t best = cand[a[i]];
+            const uint32_t pos = QuantizeAndAppend(best, rows[i], embs + (size_t)i * d);
+            row_id_cluster_mapping[rows[i]] = {best, pos};
+            CheckClusterHealth(best, allow_merges);
+        }
+    }
+
+    // index.hpp:1004-1015
+    void CheckL0ClusterHealth(uint32_t mid, bool allow_merges = true) {
+        auto& m = index.l0.clusters[mid];
+        if (m.used_capacity == m.max_capacity) {
+            if (m.num_embeddings < m.used_capacity) m.CompactCluster();
+            else SplitL0Cluster(mid);
+        } else if (allow_merges && m.num_embeddings <= m.min_capacity) {
+            DestroyAndMergeL0Cluster(mid);
+        }
+    }
+
+    // index.hpp:1017-1137: 2-means of the member L1 centroids.
+    void SplitL0Cluster(uint32_t mid) {
+        restructured_ = true;
+        const uint32_t d = index.num_dimensions;
+        const uint32_t n = index.l0.clusters[mid].num_embeddings;
+        std::vector<uint32_t> l1(index.l0.clusters[mid].indices.begin(), index.l0.clusters[mid].indices.begin() + n);
+        std::vector<float> cents((size_t)n * d);
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(&cents[(size_t)i * d], Centroid(l1[i]), d * sizeof(float));
+        auto km = host::kmeans(cents.data(), n, d, 2, kSplitIters, config.seed, 1.0f, NormalizedCentroids(), 1);
+        std::vector<uint32_t> ids[2];
+        std::vector<float> embs[2], sums[2] = {std::vector<float>(d, 0.0f), std::vector<float>(d, 0.0f)};
+        for (int g = 0; g < 2; ++g)
+            for (uint64_t i : km.assignments[g]) {
+                ids[g].push_back(l1[i]);
+                embs[g].insert(embs[g].end(), &cents[i * d], &cents[(i + 1) * d]);
+                for (uint32_t j = 0; j < d; ++j) sums[g][j] += cents[i * d + j];
+            }
+        std::vector<float> ta = CentroidMean(sums[0], ids[0].size(), km.centroids.data());
+        std::vector<float> tb = CentroidMean(sums[1], ids[1].size(), km.centroids.data() + d);
+        const uint32_t bid = index.l0.num_clusters;
+        auto make = [&](uint32_t id, int g) {
+            Cluster<F32> c(static_cast<uint32_t>(ids[g].size()), d);
+            c.id = id;
+            for (size_t i = 0; i < ids[g].size(); ++i) c.InsertEmbedding((uint32_t)i, ids[g][i], &embs[g][i * d]);
+            return c;
+        };
+        index.l0.clusters[mid] = make(mid, 0);
+        index.l0.clusters.push_back(make(bid, 1));
+        index.l0.num_clusters++;
+        std::memcpy(&index.l0.centroids[(size_t)mid * d], ta.data(), d * sizeof(float));
+        index.l0.centroids.insert(index.l0.centroids.end(), tb.begin(), tb.end());
+        for (uint32_t c : ids[0]) index.clusters[c].mesocluster_id = mid;
+        for (uint32_t c : ids[1]) index.clusters[c].mesocluster_id = bid;
+        index.l0.ComputeClusterOffsets();
+    }
+
+    // index.hpp:1139-1190. The last meso cluster is kept (its members would have nowhere to go).
+    void DestroyAndMergeL0Cluster(uint32_t mid) {
+        if (index.l0.num_clusters <= 1) return;
+        restructured_ = true;
+        const uint32_t d = index.num_dimensions;
+        auto& m = index.l0.clusters[mid];
+        m.CompactCluster();
+        const uint32_t n = m.num_embeddings;
+        std::vector<uint32_t> l1(m.indices.begin(), m.indices.begin() + n);
+        std::vector<float> cents((size_t)n * d);
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(&cents[(size_t)i * d], Centroid(l1[i]), d * sizeof(float));
+        const uint32_t last = index.l0.num_clusters - 1;
+        if (mid != last) {  // swap-and-pop
+            index.l0.clusters[mid] = std::move(index.l0.clusters[last]);
+            index.l0.clusters[mid].id = mid;
+            std::memcpy(&index.l0.centroids[(size_t)mid * d], &index.l0.centroids[(size_t)last * d], d * sizeof(float));
+            auto& mv = index.l0.clusters[mid];
+            for (uint32_t p = 0; p < mv.used_capacity; ++p)
+                if (!mv.tombstones.count(p)) index.clusters[mv.indices[p]].mesocluster_id = mid;
+        }
+        index.l0.clusters.pop_back();
+        index.l0.centroids.resize((size_t)last * d);
+        index.l0.num_clusters--;
+        index.l0.total_num_embeddings -= n;
+        ReassignEmbeddingsL0(l1.data(), cents.data(), n);
+        index.l0.ComputeClusterOffsets();
+    }
+
+    // index.hpp:1192-1239: each L1 centroid to the nearest meso centroid.
+    void ReassignEmbeddingsL0(const uint32_t* l1, const float* cents, uint32_t n) {
+        const uint32_t d = index.num_dimensions;
+        std::vector<const float*> cp;
+        for (uint32_t c = 0; c < index.l0.num_clusters; ++c) cp.push_back(&index.l0.centroids[(size_t)c * d]);
+        std::vector<uint32_t> a = Nearest(cents, n, cp);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t t = a[i];
+            index.l0.clusters[t].AppendEmbedding(l1[i], cents + (size_t)i * d);
+            index.l0.total_num_embeddings++;
+            index.clusters[l1[i]].mesocluster_id = t;
+            CheckL0ClusterHealth(t, false);
+        }
     }
 };
 
