@@ -37,6 +37,12 @@ Result to_numpy(const std::vector<PDX::KNNCandidate>& r) {
 class PyPDXIndex {
   public:
     std::unique_ptr<PDX::IPDXIndex> index;
+    int servers = 0;  // running servers hold the device copy; changes to the index wait for them to close
+
+    void require_no_server(const char* what) const {
+        if (servers)
+            throw std::runtime_error(std::string(what) + ": close the running server(s) of this index first");
+    }
 
     PyPDXIndex() = default;
     PyPDXIndex(const std::string& type, uint32_t d, uint8_t metric, uint32_t seed, uint32_t num_clusters,
@@ -69,6 +75,7 @@ class PyPDXIndex {
     }
 
     void BuildIndex(const py::array_t<float, py::array::c_style | py::array::forcecast>& data) {
+        require_no_server("build_index");
         auto b = data.request();
         if (b.ndim != 2) throw std::runtime_error("data must be a 2D numpy array (n_embeddings x dimensions)");
         if ((uint32_t)b.shape[1] != index->GetNumDimensions() && index->GetNumDimensions() != 0)
@@ -152,6 +159,7 @@ class PyPDXIndex {
     }
 
     void Append(size_t row_id, const py::array_t<float, py::array::c_style | py::array::forcecast>& e) {
+        require_no_server("append");
         auto b = e.request();
         if (b.ndim != 1) throw std::runtime_error("embedding must be a 1D numpy array");
         if ((uint32_t)b.shape[0] != index->GetNumDimensions())
@@ -167,6 +175,110 @@ class PyPDXIndex {
     }
 
     size_t DeviceBytes() const { return pdxg_index_device_bytes(index->DeviceHandle()); }
+};
+
+// Continuous batching over an index's device copy (pdxg_server_*, include/pdx_amd.h): requests of one
+// query each, with their own k and nprobe, are batched as slots free up. Not in the reference, whose
+// Python API serves one query per call.
+class PyServer {
+  public:
+    PyServer(PyPDXIndex& ix, uint32_t max_batch, uint32_t max_k, uint32_t max_nprobe, uint32_t max_slots,
+             size_t scratch_budget_bytes, uint32_t pipeline_min)
+        : owner_(&ix), d_(ix.index->GetNumDimensions()), max_k_(max_k) {
+        pdxg_server_config c{};
+        c.max_batch = max_batch;
+        c.max_k = max_k;
+        c.max_nprobe = max_nprobe;
+        c.max_slots = max_slots;
+        c.scratch_budget_bytes = scratch_budget_bytes;
+        c.pipeline_min = pipeline_min;
+        PDX::detail::check(pdxg_server_create(ix.index->DeviceHandle(), &c, &sv_));
+        owner_->servers++;
+    }
+    ~PyServer() { close(); }
+    void close() {
+        if (!sv_) return;
+        {
+            py::gil_scoped_release nogil;
+            pdxg_server_destroy(sv_);
+        }
+        sv_ = nullptr;
+        owner_->servers--;
+    }
+    void live() const {
+        if (!sv_) throw std::runtime_error("server is closed");
+    }
+    void Submit(const py::array_t<float, py::array::c_style | py::array::forcecast>& q, uint32_t knn, uint32_t nprobe,
+                uint64_t tag) {
+        live();
+        auto b = q.request();
+        if (b.ndim != 1 || (uint32_t)b.shape[0] != d_)
+            throw std::invalid_argument("query must be a 1D numpy array of " + std::to_string(d_) + " floats");
+        PDX::detail::check(pdxg_server_submit(sv_, static_cast<const float*>(b.ptr), knn, nprobe, tag));
+    }
+    // [(tag, ids, distances, latency_ms), ...] of up to max_results completed requests, waiting up to
+    // timeout_ms for the first one.
+    py::list Poll(uint32_t max_results, int timeout_ms) {
+        live();
+        std::vector<uint64_t> tags(max_results);
+        std::vector<uint32_t> ids((size_t)max_results * max_k_), cnt(max_results);
+        std::vector<float> dists((size_t)max_results * max_k_), lat(max_results);
+        uint32_t got = 0;
+        {
+            py::gil_scoped_release nogil;
+            PDX::detail::check(pdxg_server_poll(sv_, max_results, timeout_ms, tags.data(), max_k_, ids.data(),
+                                                dists.data(), cnt.data(), lat.data(), &got));
+        }
+        py::list out;
+        for (uint32_t r = 0; r < got; ++r) {
+            std::vector<PDX::KNNCandidate> res(cnt[r]);
+            for (uint32_t i = 0; i < cnt[r]; ++i) res[i] = {ids[(size_t)r * max_k_ + i], dists[(size_t)r * max_k_ + i]};
+            auto [pi, pd] = to_numpy(res);
+            out.append(py::make_tuple(tags[r], pi, pd, lat[r]));
+        }
+        return out;
+    }
+    py::dict Stats() {
+        live();
+        pdxg_server_stats st{};
+        PDX::detail::check(pdxg_server_get_stats(sv_, &st));
+        py::dict d;
+        d["submitted"] = st.submitted;
+        d["completed"] = st.completed;
+        d["batches"] = st.batches;
+        d["slots"] = st.slots;
+        d["scratch_bytes"] = st.scratch_bytes;
+        d["gpu_ms"] = st.gpu_ms;
+        d["host_ms"] = st.host_ms;
+        d["turnaround_ms"] = st.turnaround_ms;
+        return d;
+    }
+    // Scan plans per batch size from a sample of queries with their k and nprobe (the server must be
+    // idle). Returns [[largest batch, meso block, data block, two-phase meso 0/1, scan order], ...].
+    py::list Autotune(const py::array_t<float, py::array::c_style | py::array::forcecast>& q,
+                      const std::vector<uint32_t>& k, const std::vector<uint32_t>& nprobe) {
+        live();
+        auto b = q.request();
+        if (b.ndim != 2 || (uint32_t)b.shape[1] != d_) throw std::invalid_argument("queries must be n x d");
+        const uint32_t n = (uint32_t)b.shape[0];
+        if (k.size() != n || nprobe.size() != n) throw std::invalid_argument("k and nprobe need one entry per query");
+        {
+            py::gil_scoped_release nogil;
+            PDX::detail::check(pdxg_server_autotune(sv_, static_cast<const float*>(b.ptr), k.data(), nprobe.data(), n));
+        }
+        std::vector<uint32_t> rows(5 * 16);
+        uint32_t nr = 0;
+        PDX::detail::check(pdxg_server_get_plans(sv_, rows.data(), 16, &nr));
+        py::list out;
+        for (uint32_t r = 0; r < nr; ++r)
+            out.append(std::vector<uint32_t>(rows.begin() + 5 * r, rows.begin() + 5 * r + 5));
+        return out;
+    }
+
+  private:
+    PyPDXIndex* owner_;
+    uint32_t d_, max_k_;
+    pdxg_server* sv_ = nullptr;
 };
 
 }  // namespace
@@ -186,7 +298,10 @@ PYBIND11_MODULE(compiled, m) {
         .def("search_batch", &PyPDXIndex::SearchBatch, py::arg("queries"), py::arg("knn"),
              py::arg("nprobe") = py::none())
         .def("set_nprobe", [](PyPDXIndex& s, uint32_t n) { s.index->SetNProbe(n); }, py::arg("nprobe"))
-        .def("save", [](PyPDXIndex& s, const std::string& p) { s.index->Save(p); }, py::arg("path"))
+        .def("save", [](PyPDXIndex& s, const std::string& p) {
+                s.require_no_server("save");
+                s.index->Save(p);
+            }, py::arg("path"))
         .def("get_num_dimensions", [](const PyPDXIndex& s) { return s.index->GetNumDimensions(); })
         .def("get_num_clusters", [](const PyPDXIndex& s) { return s.index->GetNumClusters(); })
         .def("get_cluster_size", [](const PyPDXIndex& s, uint32_t c) { return s.index->GetClusterSize(c); },
@@ -195,6 +310,18 @@ PYBIND11_MODULE(compiled, m) {
         .def("get_in_memory_size_in_bytes", [](const PyPDXIndex& s) { return s.index->GetInMemorySizeInBytes(); })
         .def("get_device_size_in_bytes", &PyPDXIndex::DeviceBytes)
         .def("append", &PyPDXIndex::Append, py::arg("row_id"), py::arg("embedding"))
-        .def("delete", [](PyPDXIndex& s, size_t r) { s.index->Delete(r); }, py::arg("row_id"));
+        .def("delete", [](PyPDXIndex& s, size_t r) {
+                s.require_no_server("delete");
+                s.index->Delete(r);
+            }, py::arg("row_id"));
+    py::class_<PyServer>(m, "Server")
+        .def(py::init<PyPDXIndex&, uint32_t, uint32_t, uint32_t, uint32_t, size_t, uint32_t>(), py::arg("index"),
+             py::arg("max_batch") = 1024, py::arg("max_k") = 100, py::arg("max_nprobe") = 0, py::arg("max_slots") = 2,
+             py::arg("scratch_budget_bytes") = 0, py::arg("pipeline_min") = 0, py::keep_alive<1, 2>())
+        .def("submit", &PyServer::Submit, py::arg("query"), py::arg("knn"), py::arg("nprobe"), py::arg("tag"))
+        .def("poll", &PyServer::Poll, py::arg("max_results") = 1024, py::arg("timeout_ms") = 100)
+        .def("stats", &PyServer::Stats)
+        .def("autotune", &PyServer::Autotune, py::arg("queries"), py::arg("knn"), py::arg("nprobe"))
+        .def("close", &PyServer::close);
     m.def("load_index", &PyPDXIndex::Load, py::arg("path"), "Load a PDX index from a single file (auto-detects type).");
 }

@@ -12,7 +12,7 @@ try:  # the HIP runtime must be the one torch loads (same SONAME); import torch 
 except ImportError:  # pragma: no cover
     pass
 
-from pdx_amd.compiled import PDXIndex as _PDXIndex, load_index as _load_index
+from pdx_amd.compiled import PDXIndex as _PDXIndex, Server, load_index as _load_index
 
 METRIC_MAP = {"l2sq": 0, "cosine": 1, "ip": 2}
 DEFAULT_NPROBE = 32
@@ -60,6 +60,16 @@ class _IndexBase:
 
     def save(self, path: str) -> None:
         self._index.save(path)
+
+    def serve(self, *, max_batch: int = 1024, max_k: int = 100, max_nprobe: int = 0, max_slots: int = 2,
+              scratch_budget_bytes: int = 0, pipeline_min: int = 0) -> Server:
+        """A continuous-batching server over this index's device copy: ``submit(query, knn, nprobe,
+        tag)`` queues one query, ``poll(max_results, timeout_ms)`` returns completed ones as
+        ``(tag, ids, distances, latency_ms)``, ``stats()`` the counters, ``autotune(queries, knn,
+        nprobe)`` per-batch-size scan plans, ``close()`` stops it. Slots follow the scratch budget
+        (0: ``max_slots``). Append, delete, save and build raise while a server of the index runs.
+        Not in the reference, whose API serves one query per call."""
+        return Server(self._index, max_batch, max_k, max_nprobe, max_slots, scratch_budget_bytes, pipeline_min)
 
     @property
     def num_dimensions(self) -> int:

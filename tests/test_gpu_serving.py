@@ -103,3 +103,45 @@ def test_autotuned_plans_keep_results(setup):
         np.testing.assert_array_equal(ids, oi[i, :oc[i]])
         np.testing.assert_array_equal(d.view(np.uint32), od[i, :oc[i]].view(np.uint32))
     sv.close()
+
+
+@pytest.mark.parametrize("kind", ["IndexPDXIVFTreeSQ8", "IndexPDXIVF"])
+def test_python_api_server(kind, tmp_path):
+    """The server through the kept Python API (IndexPDXIVF*.serve): results equal the one-query
+    search and the oracle, counters add up, the index cannot be changed while it runs."""
+    import pdx_amd
+    from tests.pdx_testdata import mixture
+
+    x, q = mixture(4000, 128, 16, 0.3, 41, n_queries=40)
+    idx = getattr(pdx_amd, kind)(num_dimensions=128, num_clusters=40)
+    idx.build(x)
+    path = str(tmp_path / "s.pdx")
+    idx.save(path)
+    rng = np.random.default_rng(4)
+    k = rng.choice([1, 10, 20], len(q)).astype(np.uint32)
+    nprobe = rng.choice([4, 16, 0], len(q)).astype(np.uint32)
+    sv = idx.serve(max_batch=16, max_k=20, max_nprobe=0, max_slots=2)
+    assert sv.autotune(q[:8], k[:8].tolist(), nprobe[:8].tolist())
+    for i in range(len(q)):
+        sv.submit(q[i], int(k[i]), int(nprobe[i]), 100 + i)
+    got = {}
+    while len(got) < len(q):
+        for tag, ids, dists, lat in sv.poll(64, 2000):
+            assert lat > 0
+            got[tag] = (ids, dists)
+    st = sv.stats()
+    assert st["submitted"] == st["completed"] == len(q) and st["slots"] == 2
+    with pytest.raises(RuntimeError):
+        idx.save(path)                   # the server holds the device copy
+    o = OracleIndex(path)
+    oi, od, _, oc, _ = o.search(q, k, nprobe)
+    for i in range(len(q)):
+        ids, dists = got[100 + i]
+        np.testing.assert_array_equal(ids, oi[i, :oc[i]])
+        np.testing.assert_array_equal(dists.view(np.uint32), od[i, :oc[i]].view(np.uint32))
+        a_ids, a_d = idx.search(q[i], int(k[i]), nprobe=int(nprobe[i]))
+        np.testing.assert_array_equal(a_ids, ids)
+    sv.close()
+    with pytest.raises(RuntimeError):
+        sv.submit(q[0], 10, 4, 0)
+    idx.save(path)                       # allowed again once the server is closed
