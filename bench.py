@@ -670,33 +670,31 @@ def bench_dense(args, cfg, torch, _capi, builder, dev):
         rec1, err1 = measure(dn, x_all, live & (sel == 1), "after")
         flops = 2.0 * B * gemm_rows * d
         tf = flops / (stage[1] * 1e-3) / 1e12
-        sel_bytes = B * gemm_rows * 4.0
-        sel_gbs = sel_bytes / (stage[2] * 1e-3) / 1e9
         unf_tf = 2.0 * B * unf_rows * d / (st_unf[1] * 1e-3) / 1e12
         log(f"{pname}: {qps_total:.0f} QPS with updates ({qps_search:.0f} search only); stage ms "
-            f"filter/gemm/select/rerank = {stage.round(4)}; GEMM {tf:.0f} TFLOP/s over {gemm_rows:.0f} rows; "
-            f"unfiltered batch {unf_ms:.2f} ms, GEMM {unf_tf:.0f} TFLOP/s; recall after updates {rec1:.4f}")
+            f"filter/first-pass/merge/rerank = {stage.round(4)}; first pass {tf:.0f} TFLOP/s over "
+            f"{gemm_rows:.0f} rows; unfiltered batch {unf_ms:.2f} ms, first pass {unf_tf:.0f} TFLOP/s; "
+            f"recall after updates {rec1:.4f}")
         results[pname] = {
             "qps": round(qps_total, 1), "qps_search_only": round(qps_search, 1),
             "ms_per_step": round((t_search + t_upd) * 1e3 / args.steps, 4),
             "update_ms_per_step": round(t_upd * 1e3 / args.steps, 4),
             "recall_at_10_before_updates": round(rec0, 4), "recall_at_10_after_updates": round(rec1, 4),
             "distance_error_vs_fp64": {"before": err0, "after": err1},
-            "stage_ms": {"queries_and_filter": round(float(stage[0]), 4), "gemm": round(float(stage[1]), 4),
-                         "select": round(float(stage[2]), 4), "rerank": round(float(stage[3]), 4)},
+            "stage_ms": {"queries_and_filter": round(float(stage[0]), 4), "first_pass": round(float(stage[1]), 4),
+                         "merge": round(float(stage[2]), 4), "rerank": round(float(stage[3]), 4)},
             "gemm_rows_per_batch": int(gemm_rows),
             "kernels": {
-                "gemm": {"bound": "mfma", "kernel": f"hipBLASLt {pname} GEMM, FP32 accumulation",
-                         "achieved": round(tf, 1), "peak": MFMA_PEAK_TFLOPS[pname], "unit": "TFLOP/s",
-                         "frac": round(tf / MFMA_PEAK_TFLOPS[pname], 4), "ms": round(float(stage[1]), 4),
-                         "flops_per_launch": int(flops)},
-                "select": {"bound": "hbm", "kernel": "k_dense_select + k_dense_merge (top-R of the GEMM output)",
-                           "achieved": round(sel_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(sel_gbs / HBM_PEAK_GBS, 4), "ms": round(float(stage[2]), 4),
-                           "algorithmic_bytes_per_launch": int(sel_bytes)},
+                "first_pass": {"bound": "mfma",
+                               "kernel": f"k_mma_topr ({pname} MFMA GEMM, FP32 accumulation, top-R per row "
+                                         f"segment in the epilogue)",
+                               "achieved": round(tf, 1), "peak": MFMA_PEAK_TFLOPS[pname], "unit": "TFLOP/s",
+                               "frac": round(tf / MFMA_PEAK_TFLOPS[pname], 4), "ms": round(float(stage[1]), 4),
+                               "flops_per_launch": int(flops)},
             },
+            "merge_ms": round(float(stage[2]), 4),
             "unfiltered_batch": {"ms": round(unf_ms, 4), "gemm_rows": int(unf_rows),
-                                 "gemm_ms": round(float(st_unf[1]), 4), "gemm_tflops": round(unf_tf, 1),
+                                 "first_pass_ms": round(float(st_unf[1]), 4), "first_pass_tflops": round(unf_tf, 1),
                                  "stage_ms": [round(float(v), 4) for v in st_unf]},
         }
         dn.close()
@@ -732,7 +730,7 @@ def bench_dense(args, cfg, torch, _capi, builder, dev):
         "config": {"workload": workload, "global_batch": B, "parallelism": "single GPU", "precision": best},
         "recall_at_10": results[best]["recall_at_10_before_updates"],
         "precisions": results,
-        "roofline": dict(results[best]["kernels"]["gemm"], stage="gemm"),
+        "roofline": dict(results[best]["kernels"]["first_pass"], stage="first_pass"),
         "cpu_baseline": cpu,
         "build_s": round(build_s, 1),
     }

@@ -322,8 +322,8 @@ pdxg_status pdxg_dense_append(pdxg_dense* dense, const float* rows, const uint32
 pdxg_status pdxg_dense_delete(pdxg_dense* dense, const uint32_t* ids, uint32_t n);
 /* Device timing of the searches of this copy (HIP events on their stream): when enabled,
  * pdxg_dense_stage_times waits for the last search and writes the milliseconds of [0] query
- * preparation and row filtering, [1] the low-precision GEMM (summed over row chunks), [2] the
- * top-R selection, [3] the FP32 rerank. */
+ * preparation and row filtering, [1] the first pass (the low-precision GEMM with the top-R of every
+ * row segment kept in its epilogue), [2] the merge of the segments' lists, [3] the FP32 rerank. */
 pdxg_status pdxg_dense_enable_timing(pdxg_dense* dense, int enable);
 pdxg_status pdxg_dense_stage_times(pdxg_dense* dense, float* out4);
 /* [0] rows (including deleted ones), [1] row capacity, [2] id capacity, [3] device bytes,

@@ -46,7 +46,7 @@ def _run(cmd):
 def build_lib() -> str:
     src = os.path.join(CSRC, "pdx_amd.hip")
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, src, "-o", tmp, "-lhipblaslt"])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, src, "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

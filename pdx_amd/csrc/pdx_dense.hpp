@@ -2,15 +2,15 @@
 //
 // A dense copy of an F32 index holds every used slot as a row-major row in the rotated space:
 // FP32 (for the rerank), BF16 or FP8 e4m3 (for the MFMA GEMM) and the squared norm accumulated in
-// FP64. A batched search computes, per chunk of rows, S = -2 q.x with one hipBLASLt GEMM (FP32
-// accumulation), forms ||q||^2 + ||x||^2 + S, keeps the R = k * oversample smallest per query,
-// and reranks the R candidates with the exact FP32 PDX distance: the vertical fma chain over the
+// FP64. A batched search computes S = -2 q.x on the matrix cores (pdx_mma.hpp, FP32 accumulation),
+// forms ||q||^2 + ||x||^2 + S in the GEMM's epilogue, keeps the R = k * oversample smallest per
+// query, and reranks the R candidates with the exact FP32 PDX distance: the vertical fma chain over the
 // first v dims, then the 64-dim horizontal blocks in the 16-lane order -- the value the PDX scan
 // computes for a vector it does not prune (searcher.hpp:596-724 via distance_computers).
 //
 // Residual coding (when the level has centroids): the low-precision rows hold r = x - c, c the
 // centroid of the row's cluster (an appended row takes its nearest centroid), and
-// q.x = q.c + q.r with q.c from an FP32 GEMM over the centroids. r is about as small as the
+// q.x = q.c + q.r with q.c from a BF16 MFMA GEMM over the centroids. r is about as small as the
 // cluster's spread, so the low-precision error of q.r is that much smaller than the error of a
 // low-precision q.x (this is what lets FP8 e4m3 rank neighbours whose distances differ by less
 // than its rounding of whole rows).
@@ -105,30 +105,36 @@ __global__ void k_dense_max_id(const uint32_t* ids, unsigned long long nr, uint3
 }
 
 // Low-precision copy of rows [r0, r0 + nr): x (or x - c of the row's cluster when cent is given)
-// as BF16 (unscaled) or FP8 e4m3 scaled by *sx. One workgroup per row, grid-stride over rows (a
-// dispatch holds at most 2^32 work-items per dimension).
+// as BF16 (unscaled) or FP8 e4m3 scaled by *sx, in rows of ld >= d elements whose tail is zero.
+// One workgroup per row, grid-stride over rows (a dispatch holds at most 2^32 work-items per
+// dimension).
 __global__ __launch_bounds__(256) void k_dense_convert(const float* __restrict__ x32, void* xlp, unsigned long long r0,
-                                                       unsigned long long nr, uint32_t d, uint32_t prec, const float* sx,
-                                                       const float* __restrict__ cent, const uint32_t* __restrict__ rcl) {
+                                                       unsigned long long nr, uint32_t d, uint32_t ld, uint32_t prec,
+                                                       const float* sx, const float* __restrict__ cent,
+                                                       const uint32_t* __restrict__ rcl) {
     const float scale = prec == kDenseBF16 ? 1.0f : *sx;
     for (unsigned long long rr = blockIdx.x; rr < nr; rr += gridDim.x) {
         const unsigned long long r = r0 + rr;
         const float* x = x32 + r * d;
         const float* c = cent ? cent + (size_t)rcl[r] * d : nullptr;
-        for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
-            const float y = c ? x[j] - c[j] : x[j];
-            if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(xlp)[r * d + j] = __float2bfloat16(y);
-            else reinterpret_cast<__hip_fp8_storage_t*>(xlp)[r * d + j] =
+        for (uint32_t j = threadIdx.x; j < ld; j += blockDim.x) {
+            const float y = j < d ? (c ? x[j] - c[j] : x[j]) : 0.0f;
+            if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(xlp)[r * ld + j] = __float2bfloat16(y);
+            else reinterpret_cast<__hip_fp8_storage_t*>(xlp)[r * ld + j] =
                 __hip_cvt_float_to_fp8(y * scale, __HIP_SATFINITE, __HIP_E4M3);
         }
     }
 }
 
-// BF16 copy of n floats (grid-stride).
-__global__ void k_to_bf16(const float* __restrict__ src, unsigned long long n, __hip_bfloat16* __restrict__ dst) {
-    const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
-    for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step)
-        dst[e] = __float2bfloat16(src[e]);
+// BF16 copy of nr rows of d floats into rows of ld >= d elements whose tail is zero (grid-stride).
+__global__ void k_to_bf16_rows(const float* __restrict__ src, unsigned long long nr, uint32_t d, uint32_t ld,
+                               __hip_bfloat16* __restrict__ dst) {
+    const unsigned long long n = nr * ld, step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
+        const unsigned long long r = e / ld;
+        const uint32_t j = static_cast<uint32_t>(e % ld);
+        dst[e] = __float2bfloat16(j < d ? src[r * d + j] : 0.0f);
+    }
 }
 
 // FP32 squared norm of each centroid (FP64 accumulation), one workgroup per centroid.
@@ -287,13 +293,17 @@ __global__ void k_dense_bias(const float* xn, const uint8_t* live, float* bias, 
     if (r < nr) bias[r] = live[r] ? xn[r] : __builtin_inff();
 }
 
-// Queries to BF16, or to FP8 scaled by their own s_q.
-__global__ void k_dense_qconvert(const float* q, unsigned long long total, uint32_t d, uint32_t prec, const float* qsc,
-                                 void* qlp) {
+// Queries to BF16, or to FP8 scaled by their own s_q, in rows of ld >= d elements whose tail is
+// zero (total = queries x ld).
+__global__ void k_dense_qconvert(const float* q, unsigned long long total, uint32_t d, uint32_t ld, uint32_t prec,
+                                 const float* qsc, void* qlp) {
     const unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(qlp)[e] = __float2bfloat16(q[e]);
-    else reinterpret_cast<__hip_fp8_storage_t*>(qlp)[e] = __hip_cvt_float_to_fp8(q[e] * qsc[e / d], __HIP_SATFINITE, __HIP_E4M3);
+    const unsigned long long t = e / ld;
+    const uint32_t j = static_cast<uint32_t>(e % ld);
+    const float y = j < d ? q[t * d + j] : 0.0f;
+    if (prec == kDenseBF16) reinterpret_cast<__hip_bfloat16*>(qlp)[e] = __float2bfloat16(y);
+    else reinterpret_cast<__hip_fp8_storage_t*>(qlp)[e] = __hip_cvt_float_to_fp8(y * qsc[t], __HIP_SATFINITE, __HIP_E4M3);
 }
 
 // Running top-R of a stream of keys (distance bits << 32 | row) held by one wave. Keys below the
@@ -361,59 +371,7 @@ struct WaveTopR {
     }
 };
 
-// One wave per (segment, query) of one GEMM chunk: the R smallest keys of the segment
-// (distance = max(0, ||q||^2 + bias[r] + qc[q, cl[r]] + qfac[q] * S[r, q]), with qc = -2 q.c of the
-// row's cluster under residual coding, else 0; row = rowlist[r] or row0 + r), written sorted and
-// padded with all-ones keys to part[(q * nseg + seg) * R]. The query's running list from the
-// earlier chunks seeds the threshold.
-__global__ __launch_bounds__(64) void k_dense_select(const float* __restrict__ S, uint32_t m, uint32_t seg_rows,
-                                                     const float* __restrict__ bias, const uint32_t* __restrict__ rowlist,
-                                                     unsigned long long row0, const float* __restrict__ qn,
-                                                     const float* __restrict__ qfac, const float* __restrict__ qc,
-                                                     const uint32_t* __restrict__ cl, uint32_t nc,
-                                                     const uint64_t* running, const uint32_t* run_len, uint32_t R,
-                                                     uint64_t* part) {
-    __shared__ uint64_t lds[kSelBuf + kSelMax];
-    const uint32_t seg = blockIdx.x, nseg = gridDim.x, q = blockIdx.y, lane = threadIdx.x;
-    WaveTopR top;
-    top.init(lds, R);
-    if (run_len[q] == R) top.kth = running[(size_t)q * R + R - 1];
-    const uint32_t r_begin = seg * seg_rows, r_end = min(m, r_begin + seg_rows);
-    const float qq = qn[q], f = qfac[q];
-    const float* col = S + (size_t)q * m;
-    const float* qcq = qc ? qc + (size_t)q * nc : nullptr;
-    constexpr uint32_t U = 8;  // rows per lane per step, all loads issued before the offers
-    for (uint32_t r0 = r_begin; r0 < r_end; r0 += U * 64) {
-        float s[U], b[U];
-        uint32_t rr[U], row[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            rr[u] = r0 + u * 64 + lane;
-            const bool in = rr[u] < r_end;
-            s[u] = in ? col[rr[u]] : 0.0f;
-            b[u] = in ? bias[rr[u]] : __builtin_inff();
-            row[u] = rowlist ? (in ? rowlist[rr[u]] : 0u) : static_cast<uint32_t>(row0 + rr[u]);
-        }
-        if (qcq) {
-            uint32_t c[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u) c[u] = rr[u] < r_end ? cl[rr[u]] : 0u;
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                if (rr[u] < r_end) b[u] += qcq[c[u]];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const float dist = fmaxf(qq + b[u] + f * s[u], 0.0f);
-            top.offer(make_key(dist, row[u]), dist < __builtin_inff());
-        }
-    }
-    top.finish();
-    uint64_t* out = part + ((size_t)q * nseg + seg) * R;
-    for (uint32_t t = lane; t < R; t += 64) out[t] = t < top.len ? top.list()[t] : kKeyInf;
-}
-
-// One wave per query: the running list merged with the nseg partial lists of one chunk.
+// One wave per query: the running list merged with the nseg partial lists of the first pass.
 __global__ __launch_bounds__(64) void k_dense_merge(uint64_t* running, uint32_t* run_len, const uint64_t* part,
                                                     uint32_t nseg, uint32_t R) {
     __shared__ uint64_t lds[kSelBuf + kSelMax];
