@@ -79,7 +79,8 @@ struct MmaArgs {
     const uint32_t* cl;        // cluster per row (with qc)
     uint32_t seg_rows, nseg, R;
     uint64_t* lists;           // [n][nseg][kMmaCap] candidate keys
-    uint64_t* part;            // [n][nseg][R] sorted top-R per (query, segment), padded with all ones
+    uint64_t* part;            // [n][nseg][R] top-R per (query, segment), padded with all ones
+    uint64_t* bound;           // [n] per query, shared by its segments (all ones at launch), or null
 };
 
 // Swizzle of the 16-byte pieces of a stage row: piece p of row r sits in slot p ^ ((r / 4) mod 4).
@@ -283,31 +284,68 @@ __device__ __forceinline__ void wave_sort_stage(uint64_t (&k)[kMmaKPL], uint32_t
 
 __device__ __forceinline__ void wave_sort(uint64_t (&k)[kMmaKPL]) { wave_sort_stage<2>(k, threadIdx.x & 63); }
 
-// Sorts the `cnt` (<= kMmaCap) keys of one candidate list (one wave) and keeps the R smallest:
-// written back to list[0..R) when `keep_out` is null, else (padded with all ones) to keep_out[0..R).
-// Returns the wave-uniform R-th smallest key (all ones when fewer than R). Both results come from
-// registers, never from a read of what this wave has just stored.
-__device__ __forceinline__ uint64_t mma_compact(uint64_t* list, uint32_t cnt, uint32_t R, uint64_t* keep_out) {
+// Keeps the keys of one candidate list (one wave; `cnt` <= kMmaCap keys at src) that are <= bound,
+// and of those the R smallest: written to dst[0..) -- in no particular order when at most R are
+// kept, else the R smallest in ascending order -- and, with `pad`, all ones up to dst[R). src
+// and dst may be the same list: every key is in registers before the first store. Returns the
+// kept count (<= R); *kth = the largest kept key when R are kept, else the bound (wave-uniform).
+__device__ __forceinline__ uint32_t mma_prune(const uint64_t* src, uint32_t cnt, uint32_t R, uint64_t bound,
+                                              uint64_t* dst, bool pad, uint64_t* kth) {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t k[kMmaKPL];
+    bool keep[kMmaKPL];
+    uint32_t pos[kMmaKPL], total = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kMmaKPL; ++j) {
         const uint32_t e = lane * kMmaKPL + j;
-        k[j] = e < cnt ? list[e] : kKeyInf;
+        k[j] = e < cnt ? src[e] : kKeyInf;
+        keep[j] = e < cnt && k[j] <= bound;
     }
+#pragma unroll
+    for (uint32_t j = 0; j < kMmaKPL; ++j) {  // kept keys numbered key slot by key slot
+        const uint64_t b = __ballot(keep[j]);
+        pos[j] = total + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+        total += static_cast<uint32_t>(__popcll(b));
+    }
+    if (total <= R) {
+        uint64_t mx = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kMmaKPL; ++j)
+            if (keep[j]) {
+                dst[pos[j]] = k[j];
+                mx = k[j] > mx ? k[j] : mx;
+            }
+        if (pad)
+            for (uint32_t t = total + lane; t < R; t += 64) dst[t] = kKeyInf;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(mx >> 32), off) << 32) |
+                               (uint32_t)__shfl_xor((int)(uint32_t)mx, off);
+            mx = o > mx ? o : mx;
+        }
+        *kth = total == R ? mx : bound;
+        return total;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kMmaKPL; ++j) k[j] = keep[j] ? k[j] : kKeyInf;
     wave_sort(k);
     uint64_t kr = kKeyInf;
 #pragma unroll
     for (uint32_t j = 0; j < kMmaKPL; ++j) {
         const uint32_t e = lane * kMmaKPL + j;
-        if (e < R) {
-            if (keep_out) keep_out[e] = k[j];      // positions >= cnt hold all ones after the sort
-            else if (e < cnt) list[e] = k[j];
-        }
+        if (e < R) dst[e] = k[j];
         if (e == R - 1) kr = k[j];
     }
-    const uint32_t src = (R - 1) / kMmaKPL;
-    return ((uint64_t)(uint32_t)__shfl((int)(kr >> 32), src) << 32) | (uint32_t)__shfl((int)(uint32_t)kr, src);
+    const uint32_t sl = (R - 1) / kMmaKPL;
+    *kth = ((uint64_t)(uint32_t)__shfl((int)(kr >> 32), sl) << 32) | (uint32_t)__shfl((int)(uint32_t)kr, sl);
+    return R;
+}
+
+// The query's shared bound (all ones when there is none): an atomic load, so a bound another
+// workgroup is lowering is never read torn.
+__device__ __forceinline__ uint64_t mma_bound(const MmaArgs& a, uint32_t q) {
+    return a.bound ? __hip_atomic_load(a.bound + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kKeyInf;
 }
 
 // Starting accumulator of a first-pass element: its constant over the GEMM scale (-inf for a
@@ -319,9 +357,12 @@ __device__ __forceinline__ float mma_init(float base, float inv) {
 // Fused first pass: for each query of the workgroup's 256 and each row of its segment, the key of
 // dist = max(0, ||q||^2 + bias[r] (+ qc[cl[r], q]) + qfac[q] * (-2 q.x)) enters the query's
 // candidate list when it is below the query's current R-th key; a list that could not take
-// another tile is compacted to its R smallest. At the end every (query, segment) list is sorted
-// and its R smallest keys (padded with all ones) are written to part -- the exact R smallest keys
-// of the segment under (distance, row) order. Grid: (segments, query tiles of 256); with the
+// another tile is pruned. At the end each (query, segment) list's keys go to part (padded with all
+// ones). Pruning uses the smaller of the list's own R-th key and the query's shared bound, the
+// smallest R-th key any segment of the query has published: a segment holding R keys at or below
+// a key shows that key is not beaten by anything above it, so every key of the query's overall R
+// smallest stays at or below the bound, and the segments' lists together hold exactly those R
+// (under (distance, row) order) plus possibly others, which the merge discards. Grid: (segments, query tiles of 256); with the
 // segment count a multiple of 8 every query tile of a segment runs on the same XCD (workgroups
 // are dealt to the 8 XCDs round-robin), so its rows come from HBM once into that XCD's L2.
 //
@@ -415,9 +456,10 @@ __global__ __launch_bounds__(kMmaThreads) void k_mma_topr(MmaArgs a) {
         uint32_t qid[4];
 #pragma unroll
         for (uint32_t qb = 0; qb < 4; ++qb) {
-            kq[qb] = kth[wq + qb * 32 + l32e];
-            sc[qb] = qconst[2][wq + qb * 32 + l32e];
             qid[qb] = q0 + wq + qb * 32 + l32e;
+            const uint64_t kl = kth[wq + qb * 32 + l32e], gb = mma_bound(a, min(qid[qb], a.n - 1));
+            kq[qb] = gb < kl ? gb : kl;
+            sc[qb] = qconst[2][wq + qb * 32 + l32e];
         }
 #pragma unroll
         for (uint32_t rb = 0; rb < 2; ++rb) {
@@ -442,26 +484,34 @@ __global__ __launch_bounds__(kMmaThreads) void k_mma_topr(MmaArgs a) {
         }
         __threadfence_block();
         __syncthreads();
-        // lists that could not take another tile keep their R smallest (one wave per list)
+        // lists that could not take another tile keep their keys within the bound, at most the R
+        // smallest (one wave per list); a list holding R publishes its R-th key as the query's bound
         for (uint32_t ql = w; ql < kMmaQ; ql += kMmaThreads / 64) {
             if (cnt[ql] + kMmaRows > kMmaCap && q0 + ql < a.n) {
-                uint64_t* L = a.lists + ((size_t)(q0 + ql) * a.nseg + seg) * kMmaCap;
-                const uint64_t kr = mma_compact(L, cnt[ql], a.R, nullptr);
+                const uint32_t q = q0 + ql;
+                uint64_t* L = a.lists + ((size_t)q * a.nseg + seg) * kMmaCap;
+                const uint64_t kl = kth[ql], gb = mma_bound(a, q);
+                uint64_t kr;
+                const uint32_t kept = mma_prune(L, cnt[ql], a.R, gb < kl ? gb : kl, L, false, &kr);
                 if (lane == 0) {
-                    cnt[ql] = min(cnt[ql], a.R);
-                    kth[ql] = kr;  // all ones while fewer than R candidates
+                    cnt[ql] = kept;
+                    kth[ql] = kr;
+                    if (a.bound && kept == a.R) atomicMin(reinterpret_cast<unsigned long long*>(a.bound + q), kr);
                 }
             }
         }
         __threadfence_block();
         __syncthreads();
     }
-    // every list: its R smallest, sorted, into part
+    // every list: its keys within the bound, at most the R smallest, into part (padded)
     for (uint32_t ql = w; ql < kMmaQ; ql += kMmaThreads / 64) {
         const uint32_t q = q0 + ql;
         if (q >= a.n) continue;
-        mma_compact(a.lists + ((size_t)q * a.nseg + seg) * kMmaCap, cnt[ql], a.R,
-                    a.part + ((size_t)q * a.nseg + seg) * a.R);
+        const uint64_t kl = kth[ql], gb = mma_bound(a, q);
+        uint64_t kr;
+        const uint32_t kept = mma_prune(a.lists + ((size_t)q * a.nseg + seg) * kMmaCap, cnt[ql], a.R, gb < kl ? gb : kl,
+                                        a.part + ((size_t)q * a.nseg + seg) * a.R, true, &kr);
+        if (a.bound && kept == a.R && lane == 0) atomicMin(reinterpret_cast<unsigned long long*>(a.bound + q), kr);
     }
 }
 
