@@ -347,6 +347,36 @@ pdxg_status pdxg_graph_launch(pdxg_graph* graph, void* stream);
 uint32_t pdxg_graph_nodes(const pdxg_graph* graph);
 pdxg_status pdxg_graph_destroy(pdxg_graph* graph);
 
+/* ---------------------------------------------------------------------------------------------
+ * Index build on the device: the steps of BuildIndex (index.hpp:640-757) other than k-means, for
+ * a builder that keeps the rows in HBM. Every array argument is a device pointer, except the
+ * per-cluster metadata of pdxg_build_layout (host arrays); `stream` is the HIP stream (null: the
+ * default stream). Each call returns when its work is done.
+ * ------------------------------------------------------------------------------------------- */
+/* Random rotation (pruners/adsampling.hpp:44-57): Q of the FP64 Householder QR G = QR of the
+ * d x d row-major matrix g, with the sign of each diag(R) entry folded into its column of Q (so R's
+ * diagonal is positive: the unique QR of a full-rank G). Writes q32 (row-major float, the index's
+ * rotation matrix) and, when q64 is not null, the same in double. */
+pdxg_status pdxg_build_rotation(const double* g, uint32_t d, float* q32, double* q64, void* stream);
+/* y = x . m for n rows of d floats (x normalised first when `normalize`, as the search normalises
+ * a query: quantizers/scalar.hpp:23-40), with the search's own rotation kernels, so each row equals
+ * what a search computes for the same vector. minmax[0..1] = the smallest and largest value of y
+ * (the range of ScalarQuantizer::ComputeQuantizationParams, scalar.hpp:60-74). */
+pdxg_status pdxg_build_transform(const float* x, uint64_t n, uint32_t d, const float* m, int normalize, float* y,
+                                 float* minmax, void* stream);
+/* PDX layout of every cluster (layout.hpp:21-87): position p of `order` (n positions, cluster by
+ * cluster: cluster c holds positions cluster_first[c] .. cluster_first[c + 1]) puts row order[p] of
+ * y (n_rows x d) in slot p - cluster_first[c] of cluster c, whose buffer of cluster_cap[c] * d
+ * elements starts at element cluster_offset[c] of `out` (out_elems elements: bytes when u8, floats
+ * otherwise). v = vertical dims (the reference's dimension split; d - v a multiple of 64). U8
+ * stores the SQ8 codes clamp(round((y - base) * scale), 0, 255) (scalar.hpp:76-94); slots past a
+ * cluster's count are zero. cluster_first (nclusters + 1), cluster_cap and cluster_offset
+ * (nclusters) are host arrays. */
+pdxg_status pdxg_build_layout(const float* y, uint32_t d, uint32_t v, int u8, float base, float scale,
+                              const uint32_t* order, uint64_t n, uint32_t nclusters, const uint64_t* cluster_first,
+                              const uint32_t* cluster_cap, const uint64_t* cluster_offset, void* out,
+                              uint64_t out_elems, void* stream);
+
 const char* pdxg_last_error(void);
 
 /* Library / device description: "gfx950 <name> <CUs> CUs <LDS> B LDS". */

@@ -94,7 +94,8 @@ EXPORTS = ["pdxg_index_create", "pdxg_index_destroy", "pdxg_index_update_cluster
            "pdxg_dense_create", "pdxg_dense_destroy",
            "pdxg_dense_search", "pdxg_dense_append", "pdxg_dense_delete", "pdxg_dense_info",
            "pdxg_dense_enable_timing", "pdxg_dense_stage_times", "pdxg_graph_create", "pdxg_graph_launch",
-           "pdxg_graph_nodes", "pdxg_graph_destroy"]
+           "pdxg_graph_nodes", "pdxg_graph_destroy", "pdxg_build_rotation", "pdxg_build_transform",
+           "pdxg_build_layout"]
 
 _lib = None
 
@@ -168,6 +169,12 @@ def lib():
     L.pdxg_graph_nodes.restype = ctypes.c_uint32
     L.pdxg_graph_destroy.argtypes = [vp]
     L.pdxg_graph_destroy.restype = ctypes.c_int
+    L.pdxg_build_rotation.argtypes = [vp, u32, vp, vp, vp]
+    L.pdxg_build_transform.argtypes = [vp, ctypes.c_uint64, u32, vp, ctypes.c_int, vp, vp, vp]
+    L.pdxg_build_layout.argtypes = [vp, u32, u32, ctypes.c_int, ctypes.c_float, ctypes.c_float, vp, ctypes.c_uint64,
+                                    u32, u64p, u32p, u64p, vp, ctypes.c_uint64, vp]
+    for name in ("pdxg_build_rotation", "pdxg_build_transform", "pdxg_build_layout"):
+        getattr(L, name).restype = ctypes.c_int
     for name in ("pdxg_dense_create", "pdxg_dense_destroy", "pdxg_dense_search", "pdxg_dense_append",
                  "pdxg_dense_delete", "pdxg_dense_info", "pdxg_dense_enable_timing", "pdxg_dense_stage_times"):
         getattr(L, name).restype = ctypes.c_int
@@ -187,6 +194,33 @@ def lib():
 def check(status: int) -> None:
     if status != PDXG_OK:
         raise PdxgError(status, lib().pdxg_last_error().decode())
+
+
+def build_rotation(g_ptr: int, d: int, q32_ptr: int, q64_ptr: int = 0, stream: int = 0) -> None:
+    """pdxg_build_rotation on device buffers (g: d x d float64; q32 / q64: d x d outputs)."""
+    check(lib().pdxg_build_rotation(g_ptr, d, q32_ptr, q64_ptr or None, stream or None))
+
+
+def build_transform(x_ptr: int, n: int, d: int, m_ptr: int, normalize: bool, y_ptr: int, minmax_ptr: int,
+                    stream: int = 0) -> None:
+    """pdxg_build_transform on device buffers (x, y: n x d float32; m: d x d; minmax: 2 floats)."""
+    check(lib().pdxg_build_transform(x_ptr, n, d, m_ptr, 1 if normalize else 0, y_ptr, minmax_ptr, stream or None))
+
+
+def build_layout(y_ptr: int, d: int, v: int, u8: bool, base: float, scale: float, order_ptr: int, n: int,
+                 first: np.ndarray, cap: np.ndarray, offset: np.ndarray, out_ptr: int, out_elems: int,
+                 stream: int = 0) -> None:
+    """pdxg_build_layout: device rows / order / out, host per-cluster first (nc + 1), cap, offset."""
+    first = np.ascontiguousarray(first, np.uint64)
+    cap = np.ascontiguousarray(cap, np.uint32)
+    offset = np.ascontiguousarray(offset, np.uint64)
+    nc = len(cap)
+    if len(first) != nc + 1 or len(offset) != nc:
+        raise ValueError("first must have nc + 1 entries and offset nc")
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    check(lib().pdxg_build_layout(y_ptr, d, v, 1 if u8 else 0, base, scale, order_ptr, n, nc, _ptr(first, u64p),
+                                  _ptr(cap, ctypes.POINTER(ctypes.c_uint32)), _ptr(offset, u64p), out_ptr, out_elems,
+                                  stream or None))
 
 
 def device_info(device: int = 0) -> str:

@@ -8,8 +8,13 @@ floor(sqrt(nlist)) meso clusters whose members are the L1 centroids (index.hpp:7
 The reference clusters with SuperKMeans (absent from the reference tree); this module runs a
 seeded Lloyd k-means of its own (sampled training set, spherical when the data is normalised).
 Search parity does not depend on the clustering: any valid index in the reference layout is
-searched identically by the oracle and the GPU path. Heavy products run through torch on the
-GPU when one is present, numpy otherwise.
+searched identically by the oracle and the GPU path.
+
+With a GPU the build runs on the device with the rows resident in HBM: the rotation is the FP64
+Householder QR of the seeded Gaussian (pdxg_build_rotation), the rows are normalised and rotated
+by the search's own kernels with the global range reduced alongside (pdxg_build_transform),
+k-means runs on torch over the device-resident rows, and the quantised PDX layout of every
+cluster is written by pdxg_build_layout. Without one the same steps run on the host (numpy).
 """
 from __future__ import annotations
 
@@ -32,10 +37,14 @@ def compute_number_of_clusters(n: int) -> int:
     return int(math.ceil(8 * math.sqrt(n)))
 
 
+def rotation_gaussian(d: int, seed: int) -> np.ndarray:
+    """The seeded d x d standard-normal matrix whose QR gives the rotation."""
+    return np.random.default_rng(seed).standard_normal((d, d))
+
+
 def random_rotation(d: int, seed: int) -> np.ndarray:
     """Random orthogonal matrix, QR in float64 with the sign of diag(R) folded into Q."""
-    rng = np.random.default_rng(seed)
-    g = rng.standard_normal((d, d))
+    g = rotation_gaussian(d, seed)
     q, r = np.linalg.qr(g)
     q *= np.sign(np.diag(r))[None, :]
     return np.ascontiguousarray(q.astype(np.float32))
@@ -118,10 +127,11 @@ class _Backend:
             out[s:s + chunk] = r.clamp(0, 255).to(t.uint8).cpu().numpy()
         return out
 
-    def kmeans_resident(self, train: np.ndarray, k: int, iters: int, rng, angular: bool) -> np.ndarray:
-        """Lloyd iterations with the training sample resident on the GPU; returns centroids."""
+    def kmeans_resident(self, train, k: int, iters: int, rng, angular: bool) -> np.ndarray:
+        """Lloyd iterations with the training sample (host array or device tensor) resident on the
+        GPU; returns centroids."""
         t = self.torch
-        tr = t.from_numpy(np.ascontiguousarray(train)).to(self.dev)
+        tr = train if isinstance(train, t.Tensor) else t.from_numpy(np.ascontiguousarray(train)).to(self.dev)
         n, d = tr.shape
         cent = self.seed_plusplus(tr, k, rng)
         chunk = 1 << 16
@@ -187,6 +197,16 @@ class _Backend:
             out.index_add_(0, at, xt)
         return out.cpu().numpy()
 
+    def assign_resident(self, y, c: np.ndarray, chunk: int = 1 << 16) -> np.ndarray:
+        """Nearest centroid (L2, ties: lowest index) of every row of a device tensor."""
+        t = self.torch
+        ct = t.from_numpy(np.ascontiguousarray(c)).to(self.dev)
+        cnt = (ct * ct).sum(1)
+        out = t.empty(y.shape[0], dtype=t.int64, device=self.dev)
+        for s in range(0, y.shape[0], chunk):
+            out[s:s + chunk] = t.argmin(cnt[None, :] - 2.0 * (y[s:s + chunk] @ ct.T), dim=1)
+        return out.cpu().numpy()
+
     def assign(self, x: np.ndarray, c: np.ndarray, chunk: int = 1 << 16) -> np.ndarray:
         """Nearest centroid by L2 (ties: lowest index)."""
         cn = np.einsum("ij,ij->i", c, c).astype(np.float32)
@@ -211,12 +231,10 @@ def kmeans(x: np.ndarray, k: int, iters: int, seed: int, sampling_fraction: floa
            backend: _Backend):
     """Seeded Lloyd k-means; returns (centroids [k,d] float32, assignment of every row)."""
     n, d = x.shape
-    rng = np.random.default_rng(seed)
     if k == 1:
         return x[:1].copy(), np.zeros(n, np.int64)
-    frac = sampling_fraction if sampling_fraction > 0 else (1.0 if n < MIN_EMBEDDINGS_TO_SAMPLE else 0.3)
-    ns = max(k, min(n, int(n * frac)))
-    train = x[np.sort(rng.choice(n, ns, replace=False))] if ns < n else x
+    rng, rows = _sample_rows(n, k, seed, sampling_fraction)
+    train = x if rows is None else x[rows]
     if backend.torch is not None:
         cent = backend.kmeans_resident(train, k, iters, rng, angular)
         return cent, backend.assign(x, cent)
@@ -244,6 +262,72 @@ def _clusters_from_assignment(vecs, assign, k, level, row_ids):
     return clusters
 
 
+def _sample_rows(n: int, k: int, seed: int, sampling_fraction: float):
+    """(rng after drawing, sorted training rows or None for all rows) -- the draw kmeans() makes."""
+    rng = np.random.default_rng(seed)
+    frac = sampling_fraction if sampling_fraction > 0 else (1.0 if n < MIN_EMBEDDINGS_TO_SAMPLE else 0.3)
+    ns = max(k, min(n, int(n * frac)))
+    return rng, (np.sort(rng.choice(n, ns, replace=False)) if ns < n else None)
+
+
+def _build_level_on_device(x: np.ndarray, m: np.ndarray | None, d: int, seed: int, nlist: int, norm: bool, u8: bool,
+                           kmeans_iters: int, sampling_fraction: float, be: _Backend):
+    """The device build of the data level: returns (rotation, Level)."""
+    from . import _capi
+
+    t, dev = be.torch, be.dev
+    stream = t.cuda.current_stream(dev).cuda_stream
+    n = x.shape[0]
+    if m is None:
+        g = t.from_numpy(rotation_gaussian(d, seed)).to(dev)
+        mt = t.empty((d, d), dtype=t.float32, device=dev)
+        _capi.build_rotation(g.data_ptr(), d, mt.data_ptr(), 0, stream)
+        del g
+        m = mt.cpu().numpy()
+    else:
+        mt = t.from_numpy(np.ascontiguousarray(m, np.float32)).to(dev)
+    xt = t.from_numpy(x).to(dev)
+    y = t.empty((n, d), dtype=t.float32, device=dev)
+    mm = t.empty(2, dtype=t.float32, device=dev)
+    _capi.build_transform(xt.data_ptr(), n, d, mt.data_ptr(), norm, y.data_ptr(), mm.data_ptr(), stream)
+    del xt
+    lo, hi = (float(v) for v in mm.cpu().numpy())
+    h, v = pf.dimension_split(d)
+    if u8:  # ScalarQuantizer::ComputeQuantizationParams (scalar.hpp:60-74)
+        rng_ = np.float32(np.float32(hi) - np.float32(lo))
+        base, scale = float(np.float32(lo)), float(np.float32(255.0) / rng_ if rng_ > 0 else np.float32(1.0))
+    else:
+        base, scale = 0.0, 1.0
+    if nlist == 1:
+        cent, assign = y[:1].cpu().numpy(), np.zeros(n, np.int64)
+    else:
+        rng, rows = _sample_rows(n, nlist, seed, sampling_fraction)
+        train = y if rows is None else y[t.from_numpy(rows).to(dev)]
+        cent = be.kmeans_resident(train, nlist, kmeans_iters, rng, norm)
+        del train
+        assign = be.assign_resident(y, cent)
+    order = np.argsort(assign, kind="stable")
+    first = np.searchsorted(assign[order], np.arange(nlist + 1)).astype(np.uint64)
+    counts = np.diff(first).astype(np.int64)
+    caps = np.array([pf.default_capacity(int(c)) for c in counts], np.uint64)
+    offs = np.concatenate([[0], np.cumsum(caps * np.uint64(d))]).astype(np.uint64)
+    out = t.empty(int(offs[-1]), dtype=t.uint8 if u8 else t.float32, device=dev)
+    ot = t.from_numpy(order.astype(np.int32)).to(dev)
+    _capi.build_layout(y.data_ptr(), d, v, u8, base, scale, ot.data_ptr(), n, first, caps, offs[:-1],
+                       out.data_ptr(), int(offs[-1]), stream)
+    del y, ot
+    buf = out.cpu().numpy()
+    del out
+    L = pf.Level(d, v, h, u8, norm, [], cent, base, scale)
+    for c in range(nlist):
+        nc, cap, o = int(counts[c]), int(caps[c]), int(offs[c])
+        ids = np.zeros(cap, np.uint32)
+        ids[:nc] = order[first[c]:first[c + 1]]
+        L.clusters.append(pf.Cluster(n_live=nc, n_used=nc, cap=cap, data=buf[o:o + cap * d], ids=ids))
+    t.cuda.empty_cache()
+    return m, L
+
+
 def build_index(data: np.ndarray, index_type: str = "pdx_tree_u8", *, distance_metric: str = "l2sq",
                 normalize: bool = False, seed: int = 42, num_clusters: int = 0, num_meso_clusters: int = 0,
                 sampling_fraction: float = 0.0, kmeans_iters: int = 10, device=None,
@@ -259,26 +343,30 @@ def build_index(data: np.ndarray, index_type: str = "pdx_tree_u8", *, distance_m
     nlist = num_clusters or compute_number_of_clusters(n)
     norm = bool(normalize) or distance_metric in ("cosine", "ip")
     be = _Backend(device)
-    m = random_rotation(d, seed) if rotation is None else np.ascontiguousarray(rotation, np.float32)
-    y, lo, hi = be.preprocess(x, m, norm)
-    h, v = pf.dimension_split(d)
     u8 = t in (pf.TYPE_U8, pf.TYPE_TREE_U8)
-    if u8:  # ScalarQuantizer::ComputeQuantizationParams (scalar.hpp:60-74)
-        rng_ = np.float32(np.float32(hi) - np.float32(lo))
-        base, scale = float(np.float32(lo)), float(np.float32(255.0) / rng_ if rng_ > 0 else np.float32(1.0))
+    if be.torch is not None:
+        m, L = _build_level_on_device(x, rotation, d, seed, nlist, norm, u8, kmeans_iters, sampling_fraction, be)
+        cent = L.centroids
     else:
-        base, scale = 0.0, 1.0
-    cent, assign = kmeans(y, nlist, kmeans_iters, seed, sampling_fraction, norm, be)
-    L = pf.Level(d, v, h, u8, norm, [], cent, base, scale)
-    stored = be.quantize(y, base, scale) if u8 else y
-    L.clusters = _clusters_from_assignment(stored, assign, nlist, L, np.arange(n))
+        m = random_rotation(d, seed) if rotation is None else np.ascontiguousarray(rotation, np.float32)
+        y, lo, hi = be.preprocess(x, m, norm)
+        h, v = pf.dimension_split(d)
+        if u8:  # ScalarQuantizer::ComputeQuantizationParams (scalar.hpp:60-74)
+            rng_ = np.float32(np.float32(hi) - np.float32(lo))
+            base, scale = float(np.float32(lo)), float(np.float32(255.0) / rng_ if rng_ > 0 else np.float32(1.0))
+        else:
+            base, scale = 0.0, 1.0
+        cent, assign = kmeans(y, nlist, kmeans_iters, seed, sampling_fraction, norm, be)
+        L = pf.Level(d, v, h, u8, norm, [], cent, base, scale)
+        stored = be.quantize(y, base, scale) if u8 else y
+        L.clusters = _clusters_from_assignment(stored, assign, nlist, L, np.arange(n))
     idx = pf.IndexData(type=t, matrix=m, l1=L)
     if t >= 2:
         n0 = num_meso_clusters or int(math.sqrt(nlist))
         if n0 >= nlist:
             raise ValueError("num_meso_clusters must be smaller than num_clusters")
         c0, a0 = kmeans(cent, n0, 10, seed, 1.0, norm, be)
-        M = pf.Level(d, v, h, False, norm, [], c0)
+        M = pf.Level(d, L.v, L.h, False, norm, [], c0)
         M.clusters = _clusters_from_assignment(cent, a0, n0, M, np.arange(nlist))
         idx.l0 = M
     return idx
@@ -294,7 +382,22 @@ def float_twin(index: pf.IndexData, data: np.ndarray, device=None) -> pf.IndexDa
         raise ValueError("float_twin needs an SQ8 index")
     L = index.l1
     be = _Backend(device)
-    y, _, _ = be.preprocess(np.ascontiguousarray(data, np.float32), index.matrix, L.normalized)
+    x = np.ascontiguousarray(data, np.float32)
+    if be.torch is not None:  # the search's own normalise + rotate kernels
+        from . import _capi
+
+        t = be.torch
+        stream = t.cuda.current_stream(be.dev).cuda_stream
+        xt = t.from_numpy(x).to(be.dev)
+        mt = t.from_numpy(np.ascontiguousarray(index.matrix, np.float32)).to(be.dev)
+        yt = t.empty_like(xt)
+        mm = t.empty(2, dtype=t.float32, device=be.dev)
+        _capi.build_transform(xt.data_ptr(), x.shape[0], x.shape[1], mt.data_ptr(), L.normalized, yt.data_ptr(),
+                              mm.data_ptr(), stream)
+        y = yt.cpu().numpy()
+        del xt, yt
+    else:
+        y, _, _ = be.preprocess(x, index.matrix, L.normalized)
     F = pf.Level(L.d, L.v, L.h, False, L.normalized, [], L.centroids)
     for c in L.clusters:
         if c.tombstones:

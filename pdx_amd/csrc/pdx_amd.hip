@@ -1407,3 +1407,4 @@ pdxg_status pdxg_filtered_search(pdxg_index* ix, const float* query, uint32_t tr
 #include "pdx_server.inc"
 #include "pdx_dense.inc"
 #include "pdx_graph.inc"
+#include "pdx_build.inc"
