@@ -112,7 +112,10 @@ struct Level {
     std::vector<uint64_t> data_off, slot_off, tomb_off;
     uint64_t data_bytes = 0, total_slots = 0, tomb_words = 0;
     uint64_t tomb_total = 0;
-    std::vector<uint64_t> top_prefix;  // [m] = sum of the m largest n_used
+    // [m] = sum of the m largest n_used, each rounded up to a multiple of 4: the summarised
+    // vectors a query visiting m clusters may need in the two-phase meso scan (16-byte aligned
+    // component rows)
+    std::vector<uint64_t> top_prefix;
     DevBuf<uint8_t> data;
     DevBuf<uint64_t> d_data_off, d_slot_off, d_tomb_off;
     DevBuf<uint32_t> ids, d_n_used, d_n_live, d_cap, tomb, d_n_tomb;
@@ -214,6 +217,7 @@ struct Level {
         tomb_total = 0;
         for (uint32_t x : n_tomb) tomb_total += x;
         std::vector<uint32_t> sz(n_used);
+        for (uint32_t& x : sz) x = (x + 3u) & ~3u;
         std::sort(sz.begin(), sz.end(), std::greater<uint32_t>());
         top_prefix.assign(nc + 1, 0);
         for (uint32_t c = 0; c < nc; ++c) top_prefix[c + 1] = top_prefix[c] + sz[c];
@@ -571,8 +575,9 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(1024), 0, st, m);
     hipLaunchKernelGGL(k_meso_scatter, gp, dim3(kListThreads), L.nc * 4, st, m);
     hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(kWave), 0, st, m);
-    // work items: sum over clusters of ceil(entries / kMesoQ) x chunks, where a cluster's entries
-    // are at most nq and all entries at most `pairs`
+    // work items (counted on the device): sum over clusters of ceil(entries / kMesoQ) x chunks,
+    // where a cluster's entries are at most nq and all entries at most `pairs`; the grid covers
+    // this bound, one item per wave
     size_t chunks = 0, ch_max = 1;
     for (uint32_t c = 0; c < L.nc; ++c) {
         const size_t ch = std::max<size_t>(1, (L.n_used[c] + kWave - 1) / kWave);
@@ -581,11 +586,10 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     }
     const size_t items = std::min(((pairs + kMesoQ - 1) / kMesoQ + L.nc) * ch_max,
                                   ((size_t)nq + kMesoQ - 1) / kMesoQ * chunks);
-    const size_t wgs = (items + kMesoWaves - 1) / kMesoWaves;
-    m.wg_per_xcd = (uint32_t)((wgs + 7) / 8);
-    require(8ull * m.wg_per_xcd * kMesoWaves * kWave < (1ull << 32), "too many meso work items for one dispatch",
+    const size_t wpx = ((items + 7) / 8 + kMesoWaves - 1) / kMesoWaves;  // workgroups per XCD
+    require(8 * wpx * kMesoWaves * kWave < (1ull << 32), "too many meso work items for one dispatch",
             PDXG_ERR_UNSUPPORTED);
-    hipLaunchKernelGGL(k_meso_summary, dim3(8 * m.wg_per_xcd), dim3(kMesoWaves * kWave), 0, st, m);
+    hipLaunchKernelGGL(k_meso_summary, dim3((uint32_t)(8 * wpx)), dim3(kMesoWaves * kWave), 0, st, m);
     HIP_CHECK(hipGetLastError());
 }
 

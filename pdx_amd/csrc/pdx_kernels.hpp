@@ -530,6 +530,7 @@ struct Scanner {
     const uint32_t* tbm;
     const uint8_t* csel;
     const float* sb;  // SUMM: summary block of the current (query, cluster), component-major
+    uint32_t sn;      // SUMM: its component stride, n rounded up to a multiple of 4
     uint32_t n, S, ntomb;
     bool hvec;        // horizontal blocks of this cluster are 16-byte aligned
 
@@ -773,7 +774,7 @@ struct Scanner {
     // the same reduction tree (F32), or the exact integer sum (U8).
     __device__ __forceinline__ dist_t horizontal_thread(uint32_t i, uint32_t hd) const {
         if constexpr (SUMM) {
-            return sb[(size_t)(a.nw + hd / kHDim) * n + i];
+            return sb[(size_t)(a.nw + hd / kHDim) * sn + i];
         } else if constexpr (!U8) {
             const float* blk = reinterpret_cast<const float*>(base) + (size_t)(v + hd) * S + (size_t)i * kHDim;
             const float* qh = qf + v + hd;
@@ -907,8 +908,8 @@ struct Scanner {
         if constexpr (SUMM) {
             const uint32_t nw = a.nw, nh = h / kHDim;
             for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-                float dd = nw ? sb[(size_t)(nw - 1) * n + i] : 0.0f;
-                for (uint32_t b = 0; b < nh; ++b) dd += sb[(size_t)(nw + b) * n + i];
+                float dd = nw ? sb[(size_t)(nw - 1) * sn + i] : 0.0f;
+                for (uint32_t b = 0; b < nh; ++b) dd += sb[(size_t)(nw + b) * sn + i];
                 dist[i] = dd;
             }
             __syncthreads();
@@ -1047,7 +1048,7 @@ struct Scanner {
                     const uint32_t i = i0 + threadIdx.x;
                     float x[8];
 #pragma unroll
-                    for (uint32_t u = 0; u < 8; ++u) x[u] = (t0 + u < nw && i < n) ? sb[(size_t)(t0 + u) * n + i] : 0.0f;
+                    for (uint32_t u = 0; u < 8; ++u) x[u] = (t0 + u < nw && i < n) ? sb[(size_t)(t0 + u) * sn + i] : 0.0f;
 #pragma unroll
                     for (uint32_t u = 0; u < 8; ++u)
                         cnt[u] += static_cast<uint32_t>(__popcll(__ballot(t0 + u < nw && i < n && x[u] >= tau[u])));
@@ -1065,7 +1066,7 @@ struct Scanner {
             const uint32_t tw = hit ? static_cast<uint32_t>(__ffsll(static_cast<long long>(hit)) - 1) : nw - 1;
             const uint32_t prev = cpe[tw];
             bytes += (unsigned long long)n * prev * 4;
-            const float* vt = sb + (size_t)tw * n;  // just read: served by the caches
+            const float* vt = sb + (size_t)tw * sn;  // just read: served by the caches
             for (uint32_t i = threadIdx.x; i < n; i += BLOCK) dist[i] = vt[i];
             __syncthreads();
             return prev;
@@ -1108,7 +1109,7 @@ struct Scanner {
             for (uint32_t b0 = 0; b0 < nhs && alive; b0 += 16) {
                 float hv[16];
 #pragma unroll
-                for (uint32_t u = 0; u < 16; ++u) hv[u] = b0 + u < nhs ? sb[(size_t)(a.nw + b0 + u) * n + i] : 0.0f;
+                for (uint32_t u = 0; u < 16; ++u) hv[u] = b0 + u < nhs ? sb[(size_t)(a.nw + b0 + u) * sn + i] : 0.0f;
 #pragma unroll
                 for (uint32_t u = 0; u < 16; ++u) {
                     if (b0 + u < nhs && alive) {
@@ -1268,7 +1269,10 @@ struct Scanner {
             ntomb = mm.n_tomb;
             tbm = a.L.tomb + mm.tomb_off;
             hvec = ((size_t)v * S * (U8 ? 1 : 4)) % 16 == 0;
-            if constexpr (SUMM) sb = a.summ + ((size_t)q * a.region + mm.aux) * a.ns;
+            if constexpr (SUMM) {
+                sb = a.summ + ((size_t)q * a.region + mm.aux) * a.ns;
+                sn = (n + 3u) & ~3u;
+            }
             if (hl < k) {
                 if constexpr (FILT) filtered_start(k, passing);
                 else start(k);

@@ -41,7 +41,6 @@ struct MesoArgs {
     float* summ;                // per query: `region` vectors x ns floats
     unsigned long long region;
     uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector
-    uint32_t wg_per_xcd;        // summary grid = 8 x wg_per_xcd workgroups
     unsigned long long* pairs;  // optional: summarised (query, vector) pairs of the launch
 };
 
@@ -135,14 +134,15 @@ __global__ __launch_bounds__(kListThreads) void k_meso_scatter(MesoArgs a) {
         if (mm[k] != 0xFFFFFFFFu) a.list[a.lstart[mm[k]] + hist[mm[k]] + rank[k]] = val[k];
 }
 
-// Summary offsets of a query's visits: running sum of the visited clusters' sizes (one wave
-// per query; the loads of up to 4 x 64 visits are issued before their in-order scan).
+// Summary offsets of a query's visits: running sum of the visited clusters' sizes, each rounded
+// up to a multiple of 4 so that every component row of a summary block starts 16-byte aligned
+// (one wave per query; the loads of up to 4 x 64 visits are issued before their in-order scan).
 __global__ __launch_bounds__(kWave) void k_meso_soff(MesoArgs a) {
     __shared__ uint32_t red[2];
     const uint32_t q = blockIdx.x;
     const uint32_t nv = a.nvisit[q];
     const uint32_t* ord = a.order + (size_t)q * a.stride;
-    uint32_t base = 0;
+    uint32_t base = 0, pairs = 0;
     for (uint32_t r0 = 0; r0 < nv; r0 += 4 * kWave) {
         uint32_t c[4], x[4];
 #pragma unroll
@@ -156,12 +156,17 @@ __global__ __launch_bounds__(kWave) void k_meso_soff(MesoArgs a) {
         for (uint32_t u = 0; u < 4; ++u) {
             const uint32_t r = r0 + u * kWave + threadIdx.x;
             uint32_t tot;
-            const uint32_t ex = block_excl_sum<kWave>(x[u], red, &tot);
+            const uint32_t ex = block_excl_sum<kWave>((x[u] + 3u) & ~3u, red, &tot);
             if (r < nv) a.soff[(size_t)q * a.stride + r] = base + ex;
             base += tot;
+            pairs += x[u];
         }
     }
-    if (a.pairs && threadIdx.x == 0) atomicAdd(a.pairs, (unsigned long long)base);
+    if (a.pairs) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off);
+        if (threadIdx.x == 0) atomicAdd(a.pairs, (unsigned long long)pairs);
+    }
 }
 
 // Query rows are read with wave-uniform addresses through the scalar cache.
@@ -180,6 +185,7 @@ __device__ __forceinline__ void meso_summarize(const MesoArgs& a, const float* d
                                                const_float* qbase, const uint32_t (&qo)[kMesoQ],
                                                const uint32_t (&ob)[kMesoQ], uint32_t i_begin) {
     const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t sn = (n + 3u) & ~3u;  // component stride of the summary block
     const uint32_t v = a.L.v, nh = a.L.h / kHDim;
     const bool aligned = ((size_t)v * S) % 4 == 0;  // 16-byte aligned horizontal blocks
     // one 64-bit base per query and constant offsets within a step, so the query values come in
@@ -257,7 +263,7 @@ __device__ __forceinline__ void meso_summarize(const MesoArgs& a, const float* d
             if (valid) {
 #pragma unroll
                 for (uint32_t u = 0; u < NQ; ++u)
-                    if (ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)t * n + i] = acc[u];
+                    if (ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)t * sn + i] = acc[u];
             }
         }
         for (uint32_t b = 0; b < nh; ++b) {
@@ -290,20 +296,17 @@ __device__ __forceinline__ void meso_summarize(const MesoArgs& a, const float* d
                         }
                 }
                 const float hv = tree16_regs(p);
-                if (valid && ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)(a.nw + b) * n + i] = hv;
+                if (valid && ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)(a.nw + b) * sn + i] = hv;
             }
         }
     }
 }
 
-// Phase 1. Summary block of (query, rank), component-major: value t of vector i at
-// [t * n + i]; t < nw: vertical chain over dims [0, end_t); t = nw + b: horizontal block b.
-__global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
-    const uint32_t wg = (blockIdx.x % 8) * a.wg_per_xcd + blockIdx.x / 8;  // items of an XCD are contiguous
-    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint32_t item = wg * kMesoWaves + w;
+// Phase 1 work item. Summary block of (query, rank), component-major: value t of vector
+// i at [t * sn + i], sn = n rounded up to a multiple of 4; t < nw: vertical chain over dims
+// [0, end_t); t = nw + b: horizontal block b.
+__device__ __forceinline__ void meso_item(const MesoArgs& a, uint32_t item) {
     const uint32_t nc = a.L.nc;
-    if (item >= a.cstart[nc]) return;  // wave-uniform; the kernel has no barriers
     uint32_t lo = 0, hi = nc;          // cstart[lo] <= item < cstart[hi]
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -332,7 +335,8 @@ __global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
         const uint32_t q = e / a.stride;
         const uint32_t so = __builtin_amdgcn_readfirstlane(a.soff[e]);
         qo[u] = q * d;
-        ob[u] = (ok && (unsigned long long)so + n <= a.region) ? static_cast<uint32_t>(q * a.region + so) : ~0u;
+        ob[u] = (ok && (unsigned long long)so + ((n + 3u) & ~3u) <= a.region) ? static_cast<uint32_t>(q * a.region + so)
+                                                                            : ~0u;
     }
     switch (ne) {  // the last item of a cluster may hold fewer queries
         case 8: meso_summarize<8>(a, data, S, n, qbase, qo, ob, i_begin); break;
@@ -344,6 +348,20 @@ __global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
         case 2: meso_summarize<2>(a, data, S, n, qbase, qo, ob, i_begin); break;
         default: meso_summarize<1>(a, data, S, n, qbase, qo, ob, i_begin); break;
     }
+}
+
+// One item per wave. The dispatcher deals workgroups to the 8 XCDs in turn (workgroup b runs on
+// XCD b % 8), so XCD x takes the contiguous items [x * per, (x + 1) * per) of the count made on
+// the device: a cluster's items share one XCD's L2 and every XCD gets the same number of items
+// however loose the host's bound (the grid) was. Waves past the count leave at once. (A
+// grid-stride loop would need fewer waves but costs the body a register: past 80 a wave per SIMD
+// is lost, measured clearly slower.)
+__global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
+    const uint32_t total = a.cstart[a.L.nc];
+    const uint32_t per = (total + 7) / 8;
+    const uint32_t local = blockIdx.x / 8 * kMesoWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t item = blockIdx.x % 8 * per + local;
+    if (local < per && item < total) meso_item(a, item);
 }
 
 }  // namespace pdxg
