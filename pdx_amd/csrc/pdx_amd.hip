@@ -248,7 +248,7 @@ struct Level {
 
 struct LdsPlan {
     uint32_t bytes = 0;
-    uint32_t o_q, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
+    uint32_t o_q, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring, o_r16;
 };
 
 LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block, bool summ) {
@@ -277,6 +277,7 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block, bool summ) 
     // block thresholds; the replay adds its checkpoint counts and ends
     p.o_tau = take((L.h / kHDim + (L.v + kHDim - 1) / kHDim + 1 + (summ ? 2 * kMaxCheckpoints + 1 : 0)) * 4);
     p.o_ring = take(kRing * sizeof(ClusterMeta));
+    p.o_r16 = take((L.d / 16 + 1) * 4);  // ratios at multiples of 16 dims
     p.bytes = (off + 15) / 16 * 16;
     return p;
 }
@@ -476,6 +477,7 @@ void launch_scan_block(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_
     a.o_red = p.o_red;
     a.o_tau = p.o_tau;
     a.o_ring = p.o_ring;
+    a.o_r16 = p.o_r16;
     if constexpr (!FILT) {
         if (a.queue) {  // persistent: as many workgroups as are resident at once
             require(p.bytes + 64 <= (uint32_t)ix->max_lds, "no LDS left for the work-queue slot", PDXG_ERR_UNSUPPORTED);

@@ -65,7 +65,7 @@ struct ScanArgs {
     uint32_t* queue;            // optional work-queue counter (persistent launch): queries are
                                 // taken in perm order by atomically incrementing it
     // dynamic LDS layout (byte offsets)
-    uint32_t o_q, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring;
+    uint32_t o_q, o_dist, o_pos, o_h, o_hp, o_t, o_tp, o_o, o_op, o_ck, o_cp, o_red, o_tau, o_ring, o_r16;
 };
 
 // Per-cluster parameters of the next kRing visited clusters, staged in LDS in one batch so that
@@ -494,7 +494,6 @@ struct DistT<true> {
     using type = uint32_t;
 };
 
-constexpr uint32_t kVT = 4;  // vectors per thread per vertical tile
 
 // SUMM: the meso-level replay. Every partial distance the scan compares (warmup checkpoints
 // and 64-dim horizontal blocks) was computed beforehand by k_meso_summary in the same operation
@@ -505,6 +504,7 @@ struct Scanner {
     static_assert(!SUMM || (!U8 && !FILT), "the summary replay is F32 and unfiltered");
     using dist_t = typename DistT<U8>::type;
     static constexpr uint32_t G = BLOCK / 16;  // 16-lane rows per workgroup
+    static constexpr uint32_t kVT = 4;  // vectors per thread per vertical tile
 
     const ScanArgs& a;
     uint8_t* lds;
@@ -515,7 +515,9 @@ struct Scanner {
     uint64_t *H, *T, *O, *CK;
     uint32_t *HP, *TP, *OP, *CP, *red;
     dist_t* taus;     // thresholds of the current prune steps
-    const float* rat;  // ADSampling ratio table [d] (global: read a few times per cluster, cached)
+    const float* rat;  // ADSampling ratio table [d] (global)
+    float* r16;        // LDS copy of the ratios at multiples of 16 dims, where every checkpoint and,
+                       // when v is a multiple of 16, every prune step ends: [j] = rat[16 j]
     ClusterMeta* ring;
     uint32_t* wcnt;   // SUMM: pruned counts per warmup checkpoint
     uint32_t* cpe;    // warmup checkpoint ends (cumulative fetch sizes clipped to v)
@@ -551,6 +553,7 @@ struct Scanner {
         taus = reinterpret_cast<dist_t*>(lds + a.o_tau);
         rat = a.ratios;
         ring = reinterpret_cast<ClusterMeta*>(lds + a.o_ring);
+        r16 = reinterpret_cast<float*>(lds + a.o_r16);
         d = a.L.d;
         v = a.L.v;
         h = a.L.h;
@@ -576,7 +579,7 @@ struct Scanner {
     // GetPruningThreshold: adsampling.hpp:91-98 with the U8 conversion of searcher.hpp:75-91.
     __device__ __forceinline__ dist_t threshold(uint32_t k, uint32_t cur) const {
         float top = key_dist(H[k - 1]);
-        float ratio = cur == d ? 1.0f : rat[cur];
+        float ratio = cur == d ? 1.0f : (cur % 16 == 0 ? r16[cur / 16] : rat[cur]);
         float tf = top * ratio;
         if constexpr (U8) {
             float sc = tf * a.L.scale_sq;
@@ -589,6 +592,7 @@ struct Scanner {
     __device__ __forceinline__ uint32_t qbyte(uint32_t j) const { return (qw[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
 
     __device__ void load_query(uint32_t q) {
+        for (uint32_t j = threadIdx.x; j < (d + 15) / 16; j += BLOCK) r16[j] = rat[16 * j];
         if (SUMM && threadIdx.x == 0) {
             uint32_t e = 0;
             for (uint32_t t = 0; t < kMaxCheckpoints; ++t) {
@@ -720,16 +724,38 @@ struct Scanner {
         }
     }
 
-    // Vertical over dims [da, db) for one gathered slot i (VerticalPruning).
+    // Vertical over dims [da, db) for one gathered slot i (VerticalPruning). Every dim is a
+    // separate (strided) load, so 8 F32 dims or 8 U8 words are loaded before the chain uses them.
     __device__ __forceinline__ dist_t vertical_one(dist_t acc, uint32_t i, uint32_t da, uint32_t db) const {
         if constexpr (!U8) {
             const float* data = reinterpret_cast<const float*>(base);
-            for (uint32_t j = da; j < db; ++j) {
+            uint32_t j = da;
+            for (; j + 8 <= db; j += 8) {
+                float x[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) x[u] = data[(size_t)(j + u) * S + i];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const float df = qf[j + u] - x[u];
+                    acc = __builtin_fmaf(df, df, acc);
+                }
+            }
+            for (; j < db; ++j) {
                 float df = qf[j] - data[(size_t)j * S + i];
                 acc = __builtin_fmaf(df, df, acc);
             }
         } else {
             uint32_t j = da;
+            for (; j + 32 <= db; j += 32) {
+                uint32_t x[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) x[u] = *reinterpret_cast<const uint32_t*>(base + (size_t)(j + 4 * u) * S + 4 * i);
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const uint32_t w = (j >> 2) + u;
+                    acc += sqdist4(qw[w], x[u], sumsq4(qw[w]));
+                }
+            }
             for (; j + 4 <= db; j += 4) {
                 const uint32_t w = j >> 2;
                 acc += sqdist4(qw[w], *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * i), sumsq4(qw[w]));
@@ -932,17 +958,21 @@ struct Scanner {
         __syncthreads();
         if (h) {
             // one thread per vector; the block distances of a vector are independent, so their
-            // loads are issued together (two U8 blocks per round trip) and added in block order
+            // loads are issued together (U8: two or four blocks per round trip) and added in
+            // block order
             const uint32_t nhs = h / kHDim;
             for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
                 dist_t dd = dist[i];
                 if constexpr (U8) {
-                    for (uint32_t b = 0; b < nhs; b += 2) {
-                        uint32_t x0[16], x1[16];
-                        fetch_block_u8(i, b * kHDim, x0);
-                        if (b + 1 < nhs) fetch_block_u8(i, (b + 1) * kHDim, x1);
-                        dd += block_dist_u8(x0, b * kHDim);
-                        if (b + 1 < nhs) dd += block_dist_u8(x1, (b + 1) * kHDim);
+                    constexpr uint32_t NB = BLOCK == 64 ? 4 : 2;  // more vectors per thread with one wave
+                    for (uint32_t b = 0; b < nhs; b += NB) {
+                        uint32_t x[NB][16];
+#pragma unroll
+                        for (uint32_t u = 0; u < NB; ++u)
+                            if (b + u < nhs) fetch_block_u8(i, (b + u) * kHDim, x[u]);
+#pragma unroll
+                        for (uint32_t u = 0; u < NB; ++u)
+                            if (b + u < nhs) dd += block_dist_u8(x[u], (b + u) * kHDim);
                     }
                 } else {
                     for (uint32_t hd = 0; hd < h; hd += kHDim) dd += horizontal_thread(i, hd);
@@ -1312,7 +1342,7 @@ struct Scanner {
 // once the counter passes nq. A fresh Scanner per query keeps no state across the loop (the
 // persistent instantiation is separate so the one-query kernel's code is unchanged by it).
 template <bool U8, bool FILT, uint32_t BLOCK, bool SUMM = false, bool PERSIST = false>
-__global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 64 ? 4 : 1))) void k_scan(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     if constexpr (PERSIST) {
         __shared__ uint32_t next;
