@@ -104,11 +104,16 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Device stride alignment of a cluster's PDX buffer, in vectors: 32 makes every dim-major row
+// (4 bytes per vector: an F32 dim or a U8 group of 4 dims) start on a 128-byte line.
+constexpr uint32_t kStrideAlign = 32;
+
 struct Level {
     uint32_t nc = 0, d = 0, v = 0, h = 0;
     bool u8 = false;
     bool has_centroids = false;
     std::vector<uint32_t> n_used, n_live, cap, n_tomb;
+    std::vector<uint32_t> stride;  // device stride of each cluster: cap rounded up to kStrideAlign
     std::vector<uint64_t> data_off, slot_off, tomb_off;
     uint64_t data_bytes = 0, total_slots = 0, tomb_words = 0;
     uint64_t tomb_total = 0;
@@ -118,7 +123,7 @@ struct Level {
     std::vector<uint64_t> top_prefix;
     DevBuf<uint8_t> data;
     DevBuf<uint64_t> d_data_off, d_slot_off, d_tomb_off;
-    DevBuf<uint32_t> ids, d_n_used, d_n_live, d_cap, tomb, d_n_tomb;
+    DevBuf<uint32_t> ids, d_n_used, d_n_live, d_stride, tomb, d_n_tomb;
     DevBuf<float> centroids;
 
     size_t elem() const { return u8 ? 1 : 4; }
@@ -129,7 +134,7 @@ struct Level {
     }
     size_t device_bytes() const {
         return data.bytes() + d_data_off.bytes() + d_slot_off.bytes() + d_tomb_off.bytes() + ids.bytes() +
-               d_n_used.bytes() + d_n_live.bytes() + d_cap.bytes() + tomb.bytes() + d_n_tomb.bytes() +
+               d_n_used.bytes() + d_n_live.bytes() + d_stride.bytes() + tomb.bytes() + d_n_tomb.bytes() +
                centroids.bytes();
     }
 
@@ -140,15 +145,30 @@ struct Level {
         require(c.num_embeddings + c.num_tombstones == c.used_capacity,
                 "num_embeddings + tombstones must equal used_capacity");
         require(c.used_capacity == 0 || (c.data && c.ids), "cluster data/ids missing");
-        require((uint64_t)c.max_capacity * d < (1ull << 32), "cluster buffer too large");
+        require((uint64_t)(c.max_capacity + kStrideAlign) * d < (1ull << 32), "cluster buffer too large");
         for (uint32_t t = 0; t < c.num_tombstones; ++t)
             require(c.tombstones[t] < c.used_capacity, "tombstone slot out of range");
     }
 
+    // The cluster's PDX buffer (stride max_capacity) re-laid with stride[c]: each row of the layout
+    // (an F32 vertical dim, a U8 group of 4 vertical dims or the U8 tail group of v % 4 dims, a
+    // 64-dim horizontal block) keeps its per-vector layout and starts at its dim position times
+    // the new stride; the padding is zero.
     void upload_cluster(uint32_t c, const pdxg_cluster_desc& cd) {
-        const size_t bytes = (size_t)cd.max_capacity * d * elem();
         if (cd.used_capacity) {
-            HIP_CHECK(hipMemcpy(data.p + data_off[c], cd.data, bytes, hipMemcpyHostToDevice));
+            const size_t e = elem(), S = cd.max_capacity, T = stride[c];
+            std::vector<uint8_t> buf(T * d * e, 0);
+            const uint8_t* src = static_cast<const uint8_t*>(cd.data);
+            auto row = [&](size_t p, size_t w) { std::memcpy(buf.data() + p * T * e, src + p * S * e, S * w * e); };
+            if (u8) {
+                const uint32_t rem = v % 4;
+                for (uint32_t p = 0; p + 4 <= v; p += 4) row(p, 4);
+                if (rem) row(v - rem, rem);
+            } else {
+                for (uint32_t p = 0; p < v; ++p) row(p, 1);
+            }
+            for (uint32_t p = v; p < d; p += kHDim) row(p, kHDim);
+            HIP_CHECK(hipMemcpy(data.p + data_off[c], buf.data(), buf.size(), hipMemcpyHostToDevice));
             ids.upload(cd.ids, cd.used_capacity, slot_off[c]);
         }
         const uint32_t words = (cd.max_capacity + 31) / 32;
@@ -171,6 +191,7 @@ struct Level {
         n_used.resize(nc);
         n_live.resize(nc);
         cap.resize(nc);
+        stride.resize(nc);
         n_tomb.resize(nc);
         data_off.resize(nc);
         slot_off.resize(nc);
@@ -180,8 +201,9 @@ struct Level {
             const auto& cd = L.clusters[c];
             check_cluster(cd);
             cap[c] = cd.max_capacity;
+            stride[c] = (cd.max_capacity + kStrideAlign - 1) / kStrideAlign * kStrideAlign;
             data_off[c] = data_bytes;
-            data_bytes += ((size_t)cd.max_capacity * d * elem() + 255) / 256 * 256;
+            data_bytes += ((size_t)stride[c] * d * elem() + 255) / 256 * 256;
             slot_off[c] = total_slots;
             total_slots += cd.max_capacity;
             tomb_off[c] = tomb_words;
@@ -197,8 +219,8 @@ struct Level {
         d_slot_off.upload(slot_off.data(), nc);
         d_tomb_off.alloc(nc);
         d_tomb_off.upload(tomb_off.data(), nc);
-        d_cap.alloc(nc);
-        d_cap.upload(cap.data(), nc);
+        d_stride.alloc(nc);
+        d_stride.upload(stride.data(), nc);
         d_n_used.alloc(nc);
         d_n_live.alloc(nc);
         d_n_tomb.alloc(nc);
@@ -231,7 +253,7 @@ struct Level {
         x.slot_off = d_slot_off.p;
         x.n_used = d_n_used.p;
         x.n_live = d_n_live.p;
-        x.cap = d_cap.p;
+        x.stride = d_stride.p;
         x.tomb = tomb.p;
         x.tomb_off = d_tomb_off.p;
         x.n_tomb = d_n_tomb.p;
@@ -261,7 +283,7 @@ LdsPlan plan_scan_lds(const Level& L, uint32_t kmax, uint32_t block, bool summ) 
         return o;
     };
     const uint32_t nw = (L.v + 3) / 4 + L.h / 4;
-    p.o_q = take(L.u8 ? nw * 4 : L.d * 4);
+    p.o_q = take(summ ? 16 : (L.u8 ? nw * 4 : L.d * 4));  // the replay reads the query from global memory
     const uint32_t capn = L.max_used();
     p.o_dist = take(capn * 4);
     p.o_pos = take(capn * 2);  // 16-bit slots
@@ -871,17 +893,21 @@ void autotune_workspace(pdxg_index* ix, pdxg_workspace* ws, const pdxg_search_ar
         }
         // order of the per-query scans (both levels): query order, longest walk first, longest
         // first through a persistent work queue
+        // (the modes differ by a few percent at most: each keeps the best of three runs after a
+        // warm-up run)
         float best_t = 1e30f;
         uint32_t best_mode = 1;
         for (uint32_t mode = 0; mode < 3; ++mode) {
             ws->order_mode = mode;
-            float ms = 0.0f;
+            float ms = 1e30f;
             bool ok = true;
-            for (int rep = 0; rep < 2 && ok; ++rep) {
+            for (int rep = 0; rep < 4 && ok; ++rep) {
                 try {
                     search_device(ix, *ws, *args, st);
                     HIP_CHECK(hipEventSynchronize(ws->ev[4]));
-                    HIP_CHECK(hipEventElapsedTime(&ms, ws->ev[2], ws->ev[4]));
+                    float t = 0.0f;
+                    HIP_CHECK(hipEventElapsedTime(&t, ws->ev[2], ws->ev[4]));
+                    if (rep > 0) ms = std::min(ms, t);
                 } catch (const Error& e) {
                     if (e.code != PDXG_ERR_UNSUPPORTED) throw;
                     ok = false;

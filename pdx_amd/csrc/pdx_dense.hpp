@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_dense_rows(LevelDev L, const unsigned l
     const uint32_t c = blockIdx.y, i = blockIdx.x;
     const uint32_t n = L.n_used[c];
     if (i >= n) return;
-    const uint32_t S = L.cap[c], d = L.d, v = L.v;
+    const uint32_t S = L.stride[c], d = L.d, v = L.v;
     const float* data = reinterpret_cast<const float*>(L.data + L.data_off[c]);
     const unsigned long long r = row_off[c] + i;
     double acc = 0.0;

@@ -19,7 +19,9 @@ __constant__ uint32_t c_fetch[20] = {16,  16,  32,  32,  32,  32,  64,   64,   6
                                      128, 128, 128, 128, 256, 256, 512, 1024, 2048, 16384};
 
 // Device view of one IVF level (IVF<Q>, ivf_wrapper.hpp:15-179). All clusters live in one pool;
-// cluster c's PDX buffer starts at data + data_off[c] (bytes) with stride cap[c] vectors.
+// cluster c's PDX buffer starts at data + data_off[c] (bytes) with stride stride[c] vectors: its
+// max_capacity rounded up to a multiple of 32, so that every dim-major row starts on a 128-byte
+// line (the file's layout, with the cluster's max_capacity as the stride, otherwise unchanged).
 struct LevelDev {
     const uint8_t* data;
     const uint64_t* data_off;
@@ -27,7 +29,7 @@ struct LevelDev {
     const uint64_t* slot_off;  // running sum of max_capacity (cluster_offsets)
     const uint32_t* n_used;
     const uint32_t* n_live;
-    const uint32_t* cap;
+    const uint32_t* stride;
     const uint32_t* tomb;      // tombstone bitmap, cluster c at tomb_off[c] (words)
     const uint64_t* tomb_off;
     const uint32_t* n_tomb;
@@ -72,7 +74,7 @@ struct ScanArgs {
 // a cluster's scan starts without a dependent global round trip.
 struct ClusterMeta {
     unsigned long long data_off, slot_off, tomb_off;
-    uint32_t c, n_live, n_used, cap, n_tomb, aux;  // aux: passing slots (filtered) / summary offset
+    uint32_t c, n_live, n_used, stride, n_tomb, aux;  // aux: passing slots (filtered) / summary offset
 };
 constexpr uint32_t kRing = 16;
 constexpr uint32_t kPosNone = 0xFFFFu;  // a survivor slot that was pruned
@@ -508,7 +510,8 @@ struct Scanner {
 
     const ScanArgs& a;
     uint8_t* lds;
-    float* qf;        // F32 query [d]
+    const float* qf;  // F32 query [d]: in LDS, or (replay, which reads it only for a vertical
+                      // remainder after an early warmup exit) its row in global memory
     uint32_t* qw;     // U8 query words: [nvw] vertical then [h/4] horizontal
     dist_t* dist;
     uint16_t* pos;    // survivor slots (a cluster's LDS distance array keeps it far below 2^16 slots)
@@ -537,7 +540,7 @@ struct Scanner {
     bool hvec;        // horizontal blocks of this cluster are 16-byte aligned
 
     __device__ Scanner(const ScanArgs& args, uint8_t* l) : a(args), lds(l) {
-        qf = reinterpret_cast<float*>(lds + a.o_q);
+        qf = reinterpret_cast<const float*>(lds + a.o_q);
         qw = reinterpret_cast<uint32_t*>(lds + a.o_q);
         dist = reinterpret_cast<dist_t*>(lds + a.o_dist);
         pos = reinterpret_cast<uint16_t*>(lds + a.o_pos);
@@ -601,7 +604,12 @@ struct Scanner {
             }
         }
         if constexpr (!U8) {
-            for (uint32_t j = threadIdx.x; j < d; j += BLOCK) qf[j] = a.qf[(size_t)q * d + j];
+            if constexpr (SUMM) {
+                qf = a.qf + (size_t)q * d;
+            } else {
+                float* ql = reinterpret_cast<float*>(lds + a.o_q);
+                for (uint32_t j = threadIdx.x; j < d; j += BLOCK) ql[j] = a.qf[(size_t)q * d + j];
+            }
         } else {
             const uint8_t* src = a.qu + (size_t)q * d;
             for (uint32_t w = threadIdx.x; w < nvw; w += BLOCK) {
@@ -623,41 +631,42 @@ struct Scanner {
 
     // Vertical (scalar_computers.hpp:19-44 / 77-121) over dims [da, db) for the VT slots
     // i0 + t*BLOCK + tid of this thread; slots >= n are clamped to a valid slot and ignored.
-    __device__ __forceinline__ void vertical_tile(dist_t (&acc)[kVT], uint32_t i0, uint32_t da, uint32_t db) const {
-        uint32_t idx[kVT];
+    template <uint32_t VT = kVT>
+    __device__ __forceinline__ void vertical_tile(dist_t (&acc)[VT], uint32_t i0, uint32_t da, uint32_t db) const {
+        uint32_t idx[VT];
 #pragma unroll
-        for (uint32_t t = 0; t < kVT; ++t) idx[t] = min(i0 + t * BLOCK + threadIdx.x, n - 1);
+        for (uint32_t t = 0; t < VT; ++t) idx[t] = min(i0 + t * BLOCK + threadIdx.x, n - 1);
         if constexpr (!U8) {
             const float* data = reinterpret_cast<const float*>(base);
             uint32_t j = da;
-            // 8 dims x kVT vectors of loads in flight per round trip
+            // 8 dims x VT vectors of loads in flight per round trip
             for (; j + 8 <= db; j += 8) {
-                float x[8][kVT];
+                float x[8][VT];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
+                    for (uint32_t t = 0; t < VT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u) {
                     const float qj = qf[j + u];
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t) {
+                    for (uint32_t t = 0; t < VT; ++t) {
                         float df = qj - x[u][t];
                         acc[t] = __builtin_fmaf(df, df, acc[t]);
                     }
                 }
             }
             for (; j + 4 <= db; j += 4) {
-                float x[4][kVT];
+                float x[4][VT];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
+                    for (uint32_t t = 0; t < VT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     const float qj = qf[j + u];
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t) {
+                    for (uint32_t t = 0; t < VT; ++t) {
                         float df = qj - x[u][t];
                         acc[t] = __builtin_fmaf(df, df, acc[t]);
                     }
@@ -666,55 +675,56 @@ struct Scanner {
             for (; j < db; ++j) {
                 const float qj = qf[j];
 #pragma unroll
-                for (uint32_t t = 0; t < kVT; ++t) {
+                for (uint32_t t = 0; t < VT; ++t) {
                     float df = qj - data[(size_t)j * S + idx[t]];
                     acc[t] = __builtin_fmaf(df, df, acc[t]);
                 }
             }
         } else {
             uint32_t j = da;
-            // 32 dims (8 words) x kVT vectors of loads in flight per round trip
-            for (; j + 32 <= db; j += 32) {
-                uint32_t x[8][kVT];
+            // 32 dims (8 words) x VT vectors of loads in flight per round trip (VT <= 4: wider
+            // tiles take 16-dim rounds)
+            for (; VT <= 4 && j + 32 <= db; j += 32) {
+                uint32_t x[8][VT];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t)
+                    for (uint32_t t = 0; t < VT; ++t)
                         x[u][t] = *reinterpret_cast<const uint32_t*>(base + (size_t)(j + 4 * u) * S + 4 * idx[t]);
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u) {
                     const uint32_t w = (j >> 2) + u;
                     const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
+                    for (uint32_t t = 0; t < VT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
                 }
             }
             for (; j + 16 <= db; j += 16) {
-                uint32_t x[4][kVT];
+                uint32_t x[4][VT];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t)
+                    for (uint32_t t = 0; t < VT; ++t)
                         x[u][t] = *reinterpret_cast<const uint32_t*>(base + (size_t)(j + 4 * u) * S + 4 * idx[t]);
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     const uint32_t w = (j >> 2) + u;
                     const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
-                    for (uint32_t t = 0; t < kVT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
+                    for (uint32_t t = 0; t < VT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
                 }
             }
             for (; j + 4 <= db; j += 4) {
                 const uint32_t w = j >> 2;
                 const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
-                for (uint32_t t = 0; t < kVT; ++t)
+                for (uint32_t t = 0; t < VT; ++t)
                     acc[t] += sqdist4(qv, *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * idx[t]), qs);
             }
             if (j < db) {  // tail group of rem < 4 dims, stored as rem bytes per vector
                 const uint32_t rem = db - j;
 #pragma unroll
-                for (uint32_t t = 0; t < kVT; ++t)
+                for (uint32_t t = 0; t < VT; ++t)
                     for (uint32_t u = 0; u < rem; ++u) {
                         int df = static_cast<int>(qbyte(j + u)) -
                                  static_cast<int>(base[(size_t)j * S + (size_t)idx[t] * rem + u]);
@@ -1044,6 +1054,24 @@ struct Scanner {
     }
 
     // searcher.hpp:376-435. Returns the dimension index reached.
+    // One tile of a warmup step: the vertical dims [cur, end) of slots i0 + u * BLOCK + tid
+    // (u < VT); returns the wave's count of dist >= tau over the tile.
+    template <uint32_t VT>
+    __device__ __forceinline__ uint32_t warmup_tile(uint32_t i0, uint32_t cur, uint32_t end, dist_t tau) {
+        dist_t acc[VT];
+#pragma unroll
+        for (uint32_t u = 0; u < VT; ++u) acc[u] = dist[min(i0 + u * BLOCK + threadIdx.x, n - 1)];
+        vertical_tile<VT>(acc, i0, cur, end);
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < VT; ++u) {
+            uint32_t i = i0 + u * BLOCK + threadIdx.x;
+            if (i < n) dist[i] = acc[u];
+            cnt += static_cast<uint32_t>(__popcll(__ballot(i < n && acc[u] >= tau)));
+        }
+        return cnt;
+    }
+
     __device__ uint32_t warmup(uint32_t k, uint32_t passing) {
         for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
             dist_t x = dist_t(0);
@@ -1106,18 +1134,17 @@ struct Scanner {
             const uint32_t end = min(cur + c_fetch[t], v);
             const dist_t tau = threshold(k, end);
             uint32_t cnt = 0;  // wave-uniform
-            for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
-                dist_t acc[kVT];
-#pragma unroll
-                for (uint32_t u = 0; u < kVT; ++u) acc[u] = dist[min(i0 + u * BLOCK + threadIdx.x, n - 1)];
-                vertical_tile(acc, i0, cur, end);
-#pragma unroll
-                for (uint32_t u = 0; u < kVT; ++u) {
-                    uint32_t i = i0 + u * BLOCK + threadIdx.x;
-                    if (i < n) dist[i] = acc[u];
-                    cnt += static_cast<uint32_t>(__popcll(__ballot(i < n && acc[u] >= tau)));
+            bool one_round = false;
+            if constexpr (U8 && BLOCK == 64) {
+                // a 16-dim U8 step of a one-wave scan over a cluster up to twice the tile: one round
+                // of 32 loads instead of two tiles one after the other
+                if (n > BLOCK * kVT && n <= BLOCK * 2 * kVT && end - cur <= 16) {
+                    cnt = warmup_tile<2 * kVT>(0, cur, end, tau);
+                    one_round = true;
                 }
             }
+            if (!one_round)
+                for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) cnt += warmup_tile<kVT>(i0, cur, end, tau);
             npruned = block_sum_uniform<BLOCK>(cnt, red);
             bytes += (unsigned long long)n * (end - cur) * (U8 ? 1 : 4);
             cur = end;
@@ -1272,7 +1299,7 @@ struct Scanner {
                     mm.c = c;
                     mm.n_live = a.L.n_live[c];
                     mm.n_used = a.L.n_used[c];
-                    mm.cap = a.L.cap[c];
+                    mm.stride = a.L.stride[c];
                     mm.n_tomb = a.L.n_tomb[c];
                     mm.data_off = a.L.data_off[c];
                     mm.slot_off = a.L.slot_off[c];
@@ -1293,7 +1320,7 @@ struct Scanner {
             }
             if (mm.n_live == 0) continue;
             n = mm.n_used;
-            S = mm.cap;
+            S = mm.stride;
             base = a.L.data + mm.data_off;
             cid = a.L.ids + mm.slot_off;
             ntomb = mm.n_tomb;

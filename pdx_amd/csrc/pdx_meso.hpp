@@ -321,7 +321,7 @@ __device__ __forceinline__ void meso_item(const MesoArgs& a, uint32_t item) {
     const uint32_t ne = min(kMesoQ, a.lstart[m + 1] - e0);
     const uint32_t n = a.L.n_used[m];
     if (n == 0) return;
-    const uint32_t d = a.L.d, S = a.L.cap[m];
+    const uint32_t d = a.L.d, S = a.L.stride[m];
     const float* data = reinterpret_cast<const float*>(a.L.data + a.L.data_off[m]);
 
     // per query: row offset of the rotated query and first summarised vector of its block
