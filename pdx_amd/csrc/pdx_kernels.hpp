@@ -1094,6 +1094,51 @@ struct Scanner {
             if (nw == 0) return 0;
             for (uint32_t t = threadIdx.x; t < nw; t += BLOCK) wcnt[t] = 0;
             __syncthreads();
+            const uint32_t lane = threadIdx.x & (kWave - 1);
+            constexpr uint32_t RI = 3, RW = 8;  // slots per thread and checkpoints held in registers
+            if (BLOCK == 64 && n <= RI * BLOCK && nw <= RW) {  // one-wave replays (large batches)
+                // every value of the cluster in flight at once and kept: the chosen checkpoint's
+                // distances come from registers
+                float x[RI][RW];
+#pragma unroll
+                for (uint32_t r = 0; r < RI; ++r) {
+                    const uint32_t i = r * BLOCK + threadIdx.x;
+#pragma unroll
+                    for (uint32_t u = 0; u < RW; ++u) x[r][u] = (u < nw && i < n) ? sb[(size_t)u * sn + i] : 0.0f;
+                }
+                uint32_t cnt[RW];
+#pragma unroll
+                for (uint32_t u = 0; u < RW; ++u) {
+                    cnt[u] = 0;
+                    if (u < nw) {
+                        const dist_t tau = threshold(k, cpe[u]);
+#pragma unroll
+                        for (uint32_t r = 0; r < RI; ++r)
+                            cnt[u] += static_cast<uint32_t>(
+                                __popcll(__ballot(r * BLOCK + threadIdx.x < n && x[r][u] >= tau)));
+                    }
+                }
+                if (lane == 0) {
+#pragma unroll
+                    for (uint32_t u = 0; u < RW; ++u)
+                        if (u < nw && cnt[u]) atomicAdd(&wcnt[u], cnt[u]);
+                }
+                __syncthreads();
+                const uint64_t hit = __ballot(lane < nw && wcnt[lane] >= need);
+                const uint32_t tw = hit ? static_cast<uint32_t>(__ffsll(static_cast<long long>(hit)) - 1) : nw - 1;
+                const uint32_t prev = cpe[tw];
+                bytes += (unsigned long long)n * prev * 4;
+#pragma unroll
+                for (uint32_t r = 0; r < RI; ++r) {
+                    float sel = 0.0f;
+#pragma unroll
+                    for (uint32_t u = 0; u < RW; ++u) sel = u == tw ? x[r][u] : sel;
+                    const uint32_t i = r * BLOCK + threadIdx.x;
+                    if (i < n) dist[i] = sel;
+                }
+                __syncthreads();
+                return prev;
+            }
             for (uint32_t t0 = 0; t0 < nw; t0 += 8) {
                 dist_t tau[8];
                 uint32_t cnt[8];
@@ -1119,7 +1164,6 @@ struct Scanner {
             }
             __syncthreads();
             // first checkpoint whose count reaches `need` (the loop below stops there), else the last
-            const uint32_t lane = threadIdx.x & (kWave - 1);
             const uint64_t hit = __ballot(lane < nw && wcnt[lane] >= need);
             const uint32_t tw = hit ? static_cast<uint32_t>(__ffsll(static_cast<long long>(hit)) - 1) : nw - 1;
             const uint32_t prev = cpe[tw];
@@ -1242,8 +1286,50 @@ struct Scanner {
         uint64_t key[kR];
         uint32_t pay[kR];
         uint32_t alive_cnt = 0;
+        uint32_t r0 = 0;  // register slots already walked
+        if constexpr (SUMM) {
+            // no vertical remainder (the warmup ran to the last checkpoint): the stored block
+            // distances of the thread's first WS survivors are loaded together, then tested in
+            // order exactly as walk() does
+            constexpr uint32_t WS = 2, WB = 12;
+            if (BLOCK == 64 && cur == v && nhs <= WB) {  // one-wave replays (large batches)
+                float hv[WS][WB];
+                uint32_t si[WS];
+#pragma unroll
+                for (uint32_t r = 0; r < WS; ++r) {
+                    const uint32_t j = threadIdx.x + r * BLOCK;
+                    si[r] = j < np ? pos[j] : 0u;
+#pragma unroll
+                    for (uint32_t u = 0; u < WB; ++u)
+                        hv[r][u] = (j < np && u < nhs) ? sb[(size_t)(a.nw + u) * sn + si[r]] : 0.0f;
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < WS; ++r) {
+                    key[r] = kKeyInf;
+                    pay[r] = 0;
+                    if (threadIdx.x + r * BLOCK < np) {
+                        dist_t x = dist[si[r]];
+                        bool alive = true;
+#pragma unroll
+                        for (uint32_t u = 0; u < WB; ++u) {
+                            if (u < nhs && alive) {
+                                x += hv[r][u];
+                                bytes_local += kHDim * 4;
+                                alive = x < taus[u];
+                            }
+                        }
+                        if (alive) {
+                            key[r] = make_key(to_float(x), cid[si[r]]);
+                            ++alive_cnt;
+                        }
+                    }
+                }
+                r0 = WS;
+            }
+        }
 #pragma unroll
         for (uint32_t r = 0; r < kR; ++r) {
+            if (r < r0) continue;
             key[r] = kKeyInf;
             pay[r] = 0;
             const uint32_t j = threadIdx.x + r * BLOCK;
