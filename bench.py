@@ -44,7 +44,8 @@ CONFIGS = {
     "c2": dict(n=1_000_000, d=768, centers=1024, sigma=0.15, unit=True, kind="pdx_f32", nlist=1000, n_meso=0,
                normalize=False, nq=10_000, batch=1024, k=10, nprobe=32, nprobe_range=None, gt=1000),
     "c3": dict(n=5_000_000, d=1024, centers=4096, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
-               normalize=True, nq=8192, batch=4096, k=20, nprobe=0, nprobe_range=(8, 128), gt=512, f32_twin=True),
+               normalize=True, nq=8192, batch=4096, k=20, nprobe=0, nprobe_range=(8, 128), gt=512, f32_twin=True,
+               scratch_gib=3),
     # serving over a resident two-level SQ8 index; each query draws k and nprobe from the sets
     "c4": dict(n=10_000_000, d=1536, centers=8192, sigma=0.2, unit=False, kind="pdx_tree_u8", nlist=0, n_meso=0,
                normalize=True, nq=8192, batch=4096, k=100, k_choices=(10, 20, 100), nprobe=0, nprobe_range=None,
@@ -954,9 +955,12 @@ def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, spec):
     for _ in range(2):  # second pass: warm
         lat, el = sv.replay(q[:n_sat], k_q[:n_sat], nprobe_q[:n_sat], np.zeros(n_sat))
     sat = n_sat / el
-    out = {"saturated_qps": round(sat, 1), "slots": sv.stats()["slots"], "pipeline_min": pmin,
-           "max_batch": cfg["batch"], "scratch_bytes": sv.stats()["scratch_bytes"], "scratch_budget_bytes": budget,
-           "load": {}}
+    st0 = sv.stats()
+    out = {"saturated_qps": round(sat, 1), "slots": st0["slots"], "pipeline_min": pmin,
+           "max_batch": cfg["batch"], "scratch_bytes": st0["scratch_bytes"], "scratch_budget_bytes": budget,
+           "summary_capacity_bytes_per_slot": st0["summary_capacity_bytes"],
+           "saturated_mean_batch": round(st0["completed"] / max(1, st0["batches"]), 1),
+           "footprint_limited_batches": st0["footprint_limited_batches"], "load": {}}
     for frac in cfg.get("loads", (0.1, 0.5, 0.9)):
         rate = frac * sat
         n = int(min(60000, max(2000, rate * 0.4)))
@@ -967,6 +971,8 @@ def serve_leg(_capi, g, q, nprobe_q, k_q, cfg, args, plan, spec):
         s1 = sv.stats()
         nb = max(1, s1["batches"] - s0["batches"])
         per = {f: round((s1[f] - s0[f]) / nb, 4) for f in ("gpu_ms", "host_ms", "turnaround_ms")}
+        per["admitted_summary_mib"] = round((s1["admitted_summary_bytes"] - s0["admitted_summary_bytes"]) / nb / 2**20, 1)
+        per["footprint_limited"] = s1["footprint_limited_batches"] - s0["footprint_limited_batches"]
         out["load"][f"{int(frac * 100)}%"] = {"offered_qps": round(rate, 1), "served_qps": round(n / el, 1),
                                               "p50_ms": round(float(np.percentile(lat, 50)), 4),
                                               "p99_ms": round(float(np.percentile(lat, 99)), 4),

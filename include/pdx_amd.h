@@ -216,10 +216,14 @@ pdxg_status pdxg_search_meso(pdxg_index* index, const float* queries_transformed
  * pdxg_server_submit; a dispatcher thread packs the waiting requests into a batch whenever one of
  * the server's slots is free and launches it on that slot's stream, so batches overlap on the
  * device; a completion thread polls the batches' events in dispatch order and hands a batch's
- * results back once it and every batch dispatched before it are done. The number of slots follows
- * from the scratch budget: each slot owns one workspace of max_batch x max_k x max_nprobe
- * (pdxg_workspace_bytes), so admission is bounded by scratch memory. Per-query results do not
- * depend on what a query is batched with.
+ * results back once it and every batch dispatched before it are done. Admission is by scratch
+ * memory: each slot holds a fixed part sized for max_batch queries and, for two-level indexes, a
+ * buffer for the two-phase meso scan's summaries; a request's footprint there is the summarised
+ * vectors its nprobe may need (the padded sizes of the largest meso clusters it can visit), and a
+ * batch takes waiting requests in arrival order while their footprints fit. With a scratch budget,
+ * the budget is shared evenly by max_slots slots (fewer when a share would not hold the fixed part
+ * and one request of the largest footprint) and each slot's summary buffer takes what its share
+ * leaves. Per-query results do not depend on what a query is batched with.
  * ------------------------------------------------------------------------------------------- */
 typedef struct pdxg_server pdxg_server;
 
@@ -241,6 +245,12 @@ typedef struct {
     /* sums over completed batches (ms): device time of the search (HIP events around it), host
      * time from taking the requests to the last launch, and taking the requests to results ready */
     double gpu_ms, host_ms, turnaround_ms;
+    /* admission by footprint (two-level indexes with the two-phase meso scan): each slot's
+     * meso-summary buffer, the sum over dispatched batches of their requests' footprints, and the
+     * batches that the footprint closed before max_batch */
+    size_t summary_capacity_bytes;
+    double admitted_summary_bytes;
+    uint64_t footprint_limited_batches;
 } pdxg_server_stats;
 
 pdxg_status pdxg_server_create(pdxg_index* index, const pdxg_server_config* config, pdxg_server** out);

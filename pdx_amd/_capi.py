@@ -60,7 +60,9 @@ class ServerConfig(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("slots", ctypes.c_uint32), ("scratch_bytes", ctypes.c_size_t), ("gpu_ms", ctypes.c_double),
-                ("host_ms", ctypes.c_double), ("turnaround_ms", ctypes.c_double)]
+                ("host_ms", ctypes.c_double), ("turnaround_ms", ctypes.c_double),
+                ("summary_capacity_bytes", ctypes.c_size_t), ("admitted_summary_bytes", ctypes.c_double),
+                ("footprint_limited_batches", ctypes.c_uint64)]
 
 
 class SearchArgs(ctypes.Structure):

@@ -333,17 +333,23 @@ struct pdxg_workspace {
     DevBuf<uint8_t> qu8;
     DevBuf<uint32_t> k1, nv1, k0, nv0, order, ids0, cnt0;
     // two-phase meso scan (two-level indexes): partial-distance summaries of the visited meso
-    // clusters, `region` vectors per query
+    // clusters. A query's region holds the padded sizes of the meso clusters it visits; `region`
+    // bounds that for any query (the nv0_max largest clusters) and the buffer holds summ_cap
+    // summarised vectors (max_queries x region unless a server gave the slot less).
     bool summ_ok = false;   // buffers allocated
     bool summ_use = true;   // plan: use them when the index allows
     uint32_t nv0_max = 0, nw = 0, ns = 0;
-    unsigned long long region = 0;
+    unsigned long long region = 0, summ_cap = 0;
+    // optional: the batch's per-query nprobe on the host (a server's staging), so that the batch's
+    // summaries are checked against summ_cap by each query's own bound instead of `region`
+    const uint32_t* host_nprobe = nullptr;
     DevBuf<uint32_t> m_cnt, m_cur, m_lstart, m_cstart, m_list, m_soff;
+    DevBuf<unsigned long long> m_qbase;
     DevBuf<float> summ;
     size_t bytes() const {
         return perm.bytes() + queue.bytes() + qnorm.bytes() + qrot.bytes() + probe.bytes() + qu8.bytes() + k1.bytes() + nv1.bytes() + k0.bytes() +
                nv0.bytes() + order.bytes() + ids0.bytes() + cnt0.bytes() + m_cnt.bytes() + m_cur.bytes() +
-               m_lstart.bytes() + m_cstart.bytes() + m_list.bytes() + m_soff.bytes() + summ.bytes();
+               m_lstart.bytes() + m_cstart.bytes() + m_list.bytes() + m_soff.bytes() + m_qbase.bytes() + summ.bytes();
     }
 };
 
@@ -420,9 +426,10 @@ void plan_workspace(const pdxg_index* ix, uint32_t nq, uint32_t max_k, uint32_t 
         const bool halve = max_nprobe != 0 && max_nprobe < n1 / 2 && n0 / 2 != 0;
         ws.nv0_max = halve ? n0 / 2 : n0;
         ws.region = ix->l0.top_prefix[ws.nv0_max];
+        ws.summ_cap = (unsigned long long)nq * ws.region;
         ws.nw = warmup_checkpoints(ix->l0.v);
         ws.ns = ws.nw + ix->l0.h / kHDim;
-        ws.summ_ok = (size_t)nq * ws.region * ws.ns * 4 <= kMaxSummaryBytes && (uint64_t)nq * ws.region < (1ull << 32) &&
+        ws.summ_ok = (size_t)ws.summ_cap * ws.ns * 4 <= kMaxSummaryBytes && ws.summ_cap < (1ull << 32) &&
                      (uint64_t)nq * ix->d < (1ull << 32);
     }
 }
@@ -443,7 +450,8 @@ size_t workspace_bytes_for(const pdxg_index* ix, uint32_t nq, uint32_t max_k, ui
         b += 4 * ((size_t)ix->l0.nc + 1) * 4;        // counts, cursors, entry / item offsets
         b += (size_t)nq * ws.nv0_max * 4;            // entries
         b += (size_t)nq * ws.order_stride * 4;       // summary offsets
-        b += (size_t)nq * ws.region * ws.ns * 4;     // summaries
+        b += (size_t)(nq + 1) * 8;                   // region starts
+        b += (size_t)ws.summ_cap * ws.ns * 4;        // summaries
     }
     return b;
 }
@@ -474,7 +482,8 @@ void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
         ws.m_cstart.alloc(n0 + 1);
         ws.m_list.alloc(nq * ws.nv0_max);
         ws.m_soff.alloc(nq * ws.order_stride);
-        ws.summ.alloc(nq * ws.region * ws.ns);
+        ws.m_qbase.alloc(nq + 1);
+        ws.summ.alloc((size_t)ws.summ_cap * ws.ns);
     }
 }
 
@@ -565,8 +574,30 @@ void launch_scan(const pdxg_index* ix, const Level& L, ScanArgs a, uint32_t kmax
 // The two-phase meso scan runs when the workspace holds its buffers and the plan asks for it, the
 // meso level has no tombstones (the replay does not mask) and the meso clusters a query may visit
 // still fit the per-query summary region planned at workspace creation.
-bool summary_applies(const pdxg_index* ix, const pdxg_workspace& ws) {
-    return ws.summ_ok && ws.summ_use && ix->l0.tomb_total == 0 && ix->l0.top_prefix[ws.nv0_max] <= ws.region;
+// Meso clusters a query visits (k_setup's rule, index.hpp:545-566), nprobe as the call clamps it.
+uint32_t meso_visits(const pdxg_index* ix, uint32_t nprobe, uint32_t np_cap) {
+    const uint32_t n0 = ix->l0.nc, n1 = ix->l1.nc;
+    uint32_t np = nprobe;
+    if (np_cap != 0 && (np == 0 || np > np_cap)) np = np_cap;
+    const uint32_t np1 = np == 0 ? n1 : np;
+    uint32_t np0 = n0;
+    if (np1 < n1 / 2) np0 /= 2;
+    return (np0 == 0 || np0 > n0) ? n0 : np0;
+}
+
+// Summarised vectors a query may need: the padded sizes of the largest meso clusters it can visit.
+unsigned long long meso_summary_bound(const pdxg_index* ix, uint32_t nprobe, uint32_t np_cap) {
+    return ix->l0.top_prefix[meso_visits(ix, nprobe, np_cap)];
+}
+
+bool summary_applies(const pdxg_index* ix, const pdxg_workspace& ws, uint32_t nq, uint32_t np_cap) {
+    if (!(ws.summ_ok && ws.summ_use && ix->l0.tomb_total == 0)) return false;
+    if (ws.host_nprobe) {  // the batch's own bound (a server admitted it by that bound)
+        unsigned long long need = 0;
+        for (uint32_t q = 0; q < nq; ++q) need += meso_summary_bound(ix, ws.host_nprobe[q], np_cap);
+        return need <= ws.summ_cap;
+    }
+    return ix->l0.top_prefix[ws.nv0_max] <= ws.region && (unsigned long long)nq * ws.region <= ws.summ_cap;
 }
 
 void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, hipStream_t st) {
@@ -586,7 +617,8 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     m.list = ws.m_list.p;
     m.soff = ws.m_soff.p;
     m.summ = ws.summ.p;
-    m.region = ws.region;
+    m.summ_cap = ws.summ_cap;
+    m.qbase = ws.m_qbase.p;
     m.nw = ws.nw;
     m.ns = ws.ns;
     m.pairs = ws.timing && ws.stage_bytes.p ? ws.stage_bytes.p + 2 : nullptr;
@@ -599,6 +631,7 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(1024), 0, st, m);
     hipLaunchKernelGGL(k_meso_scatter, gp, dim3(kListThreads), L.nc * 4, st, m);
     hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(kWave), 0, st, m);
+    hipLaunchKernelGGL(k_meso_qbase, dim3(1), dim3(1024), 0, st, m);
     // work items (counted on the device): sum over clusters of ceil(entries / kMesoQ) x chunks,
     // where a cluster's entries are at most nq and all entries at most `pairs`; the grid covers
     // this bound, one item per wave
@@ -785,14 +818,14 @@ void search_device(pdxg_index* ix, pdxg_workspace& ws, const pdxg_search_args& a
         s0.out_stride = ws.k0_max;
         s0.stage_bytes = sb;
         s0.queue = queues[0];
-        const bool summ = summary_applies(ix, ws);
+        const bool summ = summary_applies(ix, ws, nq, np_cap);
         ws.last_summ = summ;
         if (summ) {
             launch_meso_summary(ix, ws, nq, st);
             if (ws.timing) (void)hipEventRecord(ws.ev_summ, st);
             s0.summ = ws.summ.p;
             s0.soff = ws.m_soff.p;
-            s0.region = ws.region;
+            s0.qbase = ws.m_qbase.p;
             s0.nw = ws.nw;
             s0.ns = ws.ns;
             launch_scan<false, false, true>(ix, ix->l0, s0, ws.k0_max, st, ws.block[0]);
@@ -1140,7 +1173,7 @@ pdxg_status pdxg_workspace_set_order(pdxg_workspace* ws, int mode) {
 int pdxg_workspace_order(const pdxg_workspace* ws) { return ws ? static_cast<int>(ws->order_mode) : -1; }
 
 int pdxg_workspace_meso_summary(const pdxg_workspace* ws) {
-    return ws && ws->index && summary_applies(ws->index, *ws) ? 1 : 0;
+    return ws && ws->index && summary_applies(ws->index, *ws, ws->max_queries, ws->max_nprobe) ? 1 : 0;
 }
 
 pdxg_status pdxg_workspace_get_plan(pdxg_workspace* ws, uint32_t* out2) {

@@ -58,9 +58,9 @@ struct ScanArgs {
     const uint8_t* sel;         // filtered search: selection byte per slot (cluster at slot_off)
     const uint32_t* n_pass;     // filtered search: passing slots per cluster
     // two-phase meso scan: per (query, visited cluster) partial-distance summaries
-    const float* summ;          // regions of `region` vectors x ns floats per query
+    const float* summ;          // per query a region of its visited clusters' padded sizes x ns floats
     const uint32_t* soff;       // [nq x order_stride] first summarised vector of (query, rank)
-    unsigned long long region;
+    const unsigned long long* qbase;  // [nq + 1] first summarised vector of each query's region
     uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector (nw + h/64)
     uint32_t nq;
     const uint32_t* perm;       // optional launch order: workgroup b runs query perm[b]
@@ -1413,7 +1413,7 @@ struct Scanner {
             tbm = a.L.tomb + mm.tomb_off;
             hvec = ((size_t)v * S * (U8 ? 1 : 4)) % 16 == 0;
             if constexpr (SUMM) {
-                sb = a.summ + ((size_t)q * a.region + mm.aux) * a.ns;
+                sb = a.summ + ((size_t)a.qbase[q] + mm.aux) * a.ns;
                 sn = (n + 3u) & ~3u;
             }
             if (hl < k) {

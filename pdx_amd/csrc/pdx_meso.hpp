@@ -38,8 +38,10 @@ struct MesoArgs {
                                 // (8-entry group, 64-vector chunk) pairs, chunk-minor
     uint32_t* list;             // entries q * stride + rank, grouped by cluster
     uint32_t* soff;             // [nq x stride] first summarised vector of (query, rank) in its region
-    float* summ;                // per query: `region` vectors x ns floats
-    unsigned long long region;
+    unsigned long long* qbase;  // [nq + 1] first summarised vector of each query's region (the
+                                // running sum of the queries' padded visited-cluster sizes)
+    float* summ;                // summ_cap vectors x ns floats
+    unsigned long long summ_cap;
     uint32_t nw, ns;            // warmup checkpoints; floats per summarised vector
     unsigned long long* pairs;  // optional: summarised (query, vector) pairs of the launch
 };
@@ -137,6 +139,7 @@ __global__ __launch_bounds__(kListThreads) void k_meso_scatter(MesoArgs a) {
 // Summary offsets of a query's visits: running sum of the visited clusters' sizes, each rounded
 // up to a multiple of 4 so that every component row of a summary block starts 16-byte aligned
 // (one wave per query; the loads of up to 4 x 64 visits are issued before their in-order scan).
+// The query's total goes to qbase[q]; k_meso_qbase turns those into region starts.
 __global__ __launch_bounds__(kWave) void k_meso_soff(MesoArgs a) {
     __shared__ uint32_t red[2];
     const uint32_t q = blockIdx.x;
@@ -162,11 +165,28 @@ __global__ __launch_bounds__(kWave) void k_meso_soff(MesoArgs a) {
             pairs += x[u];
         }
     }
+    if (threadIdx.x == 0) a.qbase[q] = base;
     if (a.pairs) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off);
         if (threadIdx.x == 0) atomicAdd(a.pairs, (unsigned long long)pairs);
     }
+}
+
+// Region starts: exclusive running sum of the per-query totals in qbase[0 .. nq), in place, and
+// the grand total in qbase[nq] (one workgroup of 1024).
+__global__ __launch_bounds__(1024) void k_meso_qbase(MesoArgs a) {
+    __shared__ uint32_t red[1024 / kWave + 1];
+    unsigned long long base = 0;
+    for (uint32_t q0 = 0; q0 < a.nq; q0 += 1024) {
+        const uint32_t q = q0 + threadIdx.x;
+        const uint32_t x = q < a.nq ? static_cast<uint32_t>(a.qbase[q]) : 0u;  // one query's total < 2^32
+        uint32_t tot;
+        const uint32_t ex = block_excl_sum<1024>(x, red, &tot);
+        if (q < a.nq) a.qbase[q] = base + ex;
+        base += tot;
+    }
+    if (threadIdx.x == 0) a.qbase[a.nq] = base;
 }
 
 // Query rows are read with wave-uniform addresses through the scalar cache.
@@ -335,8 +355,10 @@ __device__ __forceinline__ void meso_item(const MesoArgs& a, uint32_t item) {
         const uint32_t q = e / a.stride;
         const uint32_t so = __builtin_amdgcn_readfirstlane(a.soff[e]);
         qo[u] = q * d;
-        ob[u] = (ok && (unsigned long long)so + ((n + 3u) & ~3u) <= a.region) ? static_cast<uint32_t>(q * a.region + so)
-                                                                            : ~0u;
+        // the query's region is [qbase[q], qbase[q + 1]) and lies within the buffer (the host
+        // admitted the batch by a bound on it); checked again so no store can leave either
+        const unsigned long long r0 = a.qbase[q], r1 = a.qbase[q + 1];
+        ob[u] = (ok && r0 + so + ((n + 3u) & ~3u) <= r1 && r1 <= a.summ_cap) ? static_cast<uint32_t>(r0 + so) : ~0u;
     }
     switch (ne) {  // the last item of a cluster may hold fewer queries
         case 8: meso_summarize<8>(a, data, S, n, qbase, qo, ob, i_begin); break;

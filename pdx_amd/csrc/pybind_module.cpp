@@ -251,6 +251,9 @@ class PyServer {
         d["gpu_ms"] = st.gpu_ms;
         d["host_ms"] = st.host_ms;
         d["turnaround_ms"] = st.turnaround_ms;
+        d["summary_capacity_bytes"] = st.summary_capacity_bytes;
+        d["admitted_summary_bytes"] = st.admitted_summary_bytes;
+        d["footprint_limited_batches"] = st.footprint_limited_batches;
         return d;
     }
     // Scan plans per batch size from a sample of queries with their k and nprobe (the server must be

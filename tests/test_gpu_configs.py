@@ -80,17 +80,23 @@ def test_c4_regime_served_tree_sq8_d1536():
     rng = np.random.default_rng(62)
     k = rng.choice(np.array([10, 20, 100], np.uint32), len(q))
     nprobe = rng.choice(np.array([16, 64, 256], np.uint32), len(q))
-    probe = _capi.Server(g, max_batch=64, max_k=100, max_nprobe=256, max_slots=8)
-    per_slot = probe.stats()["scratch_bytes"] // probe.stats()["slots"]
+    probe = _capi.Server(g, max_batch=64, max_k=100, max_nprobe=256, max_slots=1)
+    st = probe.stats()
     probe.close()
-    sv = _capi.Server(g, max_batch=64, max_k=100, max_nprobe=256, max_slots=8, scratch_budget_bytes=3 * per_slot)
-    assert sv.stats()["slots"] == 3
+    per_query = st["summary_capacity_bytes"] // 64   # the largest footprint of one request
+    fixed = st["scratch_bytes"] - st["summary_capacity_bytes"]
+    # three slots whose meso-summary buffers hold ten requests of the largest footprint each:
+    # admission by footprint closes the batches of this burst before max_batch
+    sv = _capi.Server(g, max_batch=64, max_k=100, max_nprobe=256, max_slots=3,
+                      scratch_budget_bytes=3 * (fixed + 10 * per_query))
+    assert sv.stats()["slots"] == 3 and sv.stats()["summary_capacity_bytes"] == 10 * per_query
     for i in range(len(q)):
         sv.submit(q[i], int(k[i]), int(nprobe[i]), i)
     got = {}
     while len(got) < len(q):
         for tag, ids, d, lat in sv.poll(256, 2000):
             got[tag] = (ids, d)
+    assert sv.stats()["footprint_limited_batches"] > 0
     sv.close()
     o = OracleIndex(data=idx)
     oi, od, _, oc, _ = o.search(q, k, nprobe, threads=16)

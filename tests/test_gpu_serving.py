@@ -69,19 +69,66 @@ def test_replay_returns_every_result_with_latency(setup):
 def test_scratch_budget_bounds_the_slots(setup):
     _capi, path, g, qs, k, nprobe = setup
     one = _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8)
-    per_slot = one.stats()["scratch_bytes"] // one.stats()["slots"]
+    st = one.stats()
     one.close()
-    two = _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8, scratch_budget_bytes=2 * per_slot + 1)
-    assert two.stats()["slots"] == 2
+    per_slot = st["scratch_bytes"] // st["slots"]
+    per_query = st["summary_capacity_bytes"] // 32      # the largest footprint of one request
+    fixed = per_slot - st["summary_capacity_bytes"]
+    # a budget for two slots of one request each shares out as two slots
+    two = _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8,
+                       scratch_budget_bytes=2 * (fixed + per_query) + 1)
+    assert two.stats()["slots"] == 2 and two.stats()["summary_capacity_bytes"] == per_query
     two.close()
+    # a generous budget: max_slots slots, each with whole summary buffers
+    many = _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=3, scratch_budget_bytes=4 * per_slot)
+    assert many.stats()["slots"] == 3 and many.stats()["summary_capacity_bytes"] == st["summary_capacity_bytes"]
+    many.close()
     with pytest.raises(_capi.PdxgError):
-        _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8, scratch_budget_bytes=per_slot // 2)
+        _capi.Server(g, max_batch=32, max_k=20, max_nprobe=16, max_slots=8, scratch_budget_bytes=fixed // 2)
     s = _capi.Server(g, max_batch=4, max_k=20, max_nprobe=16, max_slots=1)
     with pytest.raises(_capi.PdxgError):
         s.submit(qs[0], 50, 4, 0)       # k above max_k
     with pytest.raises(_capi.PdxgError):
         s.submit(qs[0], 10, 0, 0)       # nprobe "all" above max_nprobe
     s.close()
+
+
+def test_admission_by_footprint(setup):
+    """A budget below one whole slot leaves one slot whose meso-summary buffer holds three requests
+    of the largest footprint (nprobe "all": every meso cluster): batches close by footprint before
+    max_batch, smaller footprints (nprobe 4: half the meso clusters) fit more per batch, and every
+    result still equals the oracle's."""
+    _capi, path, g, qs, k, nprobe = setup
+    free = _capi.Server(g, max_batch=16, max_k=100, max_nprobe=0, max_slots=1)
+    st = free.stats()
+    free.close()
+    per_query = st["summary_capacity_bytes"] // 16
+    fixed = st["scratch_bytes"] - st["summary_capacity_bytes"]
+    assert per_query > 0 and fixed > 0
+    sv = _capi.Server(g, max_batch=16, max_k=100, max_nprobe=0, max_slots=1, scratch_budget_bytes=fixed + 3 * per_query)
+    st = sv.stats()
+    assert st["slots"] == 1 and st["summary_capacity_bytes"] == 3 * per_query
+    o = OracleIndex(path)
+    n = len(qs)
+    for np_all in (0, 4):
+        npb = np.full(n, np_all, np.uint32)
+        before = sv.stats()
+        for i in range(n):
+            sv.submit(qs[i], int(k[i]), np_all, i)
+        got = _collect(sv, n)
+        after = sv.stats()
+        batches = after["batches"] - before["batches"]
+        admitted = after["admitted_summary_bytes"] - before["admitted_summary_bytes"]
+        assert admitted <= batches * st["summary_capacity_bytes"]
+        if np_all == 0:
+            assert batches >= -(-n // 3)
+            assert after["footprint_limited_batches"] > before["footprint_limited_batches"]
+        oi, od, _, oc, _ = o.search(qs, k, npb)
+        for i in range(n):
+            ids, d, _ = got[i]
+            np.testing.assert_array_equal(ids, oi[i, :oc[i]])
+            np.testing.assert_array_equal(d.view(np.uint32), od[i, :oc[i]].view(np.uint32))
+    sv.close()
 
 
 def test_autotuned_plans_keep_results(setup):
