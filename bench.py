@@ -149,6 +149,9 @@ def main():
     ap.add_argument("--index-cache", default="",
                     help="index file: loaded when present, else written after the build (e.g. to profile without "
                          "the build's kernels)")
+    ap.add_argument("--slots", type=int, default=0,
+                    help="batches in flight in the timed leg, each on its own stream and workspace (0: the best of "
+                         "1-4 on a short trial)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -315,7 +318,7 @@ def main():
     if n_par:
         parity = check_parity(idx, q, nprobe_q, k_q, n_par, run, out_ids, out_d, out_i, out_c, torch)
 
-    # timed leg: `steps` batches, inputs resident in HBM, one batch in flight at a time
+    # one batch in flight at a time: stage and kernel times (HIP events on the search stream)
     nb = max(1, NQ // B)
     for i in range(args.warmup):
         run((i % nb) * B, B)
@@ -334,16 +337,30 @@ def main():
             meso += ws.meso_times()
             pairs += ws.meso_pairs()
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
-    qps = B * args.steps / elapsed
+    solo_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     stage /= args.steps
     sbytes /= args.steps
     meso /= args.steps
     pairs /= args.steps
-    log(f"batched: {ms_per_step:.3f} ms/step, {qps:.0f} QPS; stage ms preprocess/probe/meso/scan = {stage.round(4)}; "
-        f"scan bytes meso/data = {sbytes[0]:.3e}/{sbytes[1]:.3e}"
+    log(f"one batch in flight: {solo_ms:.3f} ms/batch, {B / solo_ms * 1e3:.0f} QPS; stage ms "
+        f"preprocess/probe/meso/scan = {stage.round(4)}; scan bytes meso/data = {sbytes[0]:.3e}/{sbytes[1]:.3e}"
         + (f"; meso phase1/replay ms = {meso.round(4)}, pairs {pairs:.4g}" if tree else ""))
+
+    # timed leg: `steps` batches with S in flight, each slot a stream and a workspace of its own
+    # (the tuned plan), so that one batch's HBM-bound data scan runs beside another's VALU-bound
+    # meso phase; inputs resident in HBM, the device synchronised on both sides
+    def slot_search(w, s0, o, stream):
+        w.search(qd[s0:s0 + B].data_ptr(), B, kd[s0:s0 + B].data_ptr() if per_k else 0,
+                 state["np_dev"][s0:s0 + B].data_ptr(), K, state["np_max"], K, o[0].data_ptr(), o[1].data_ptr(),
+                 o[2].data_ptr(), o[3].data_ptr(), 0, stream=stream, k_all=K)
+
+    ov = Overlap(torch, dev, ws, (out_ids, out_d, out_i, out_c), lambda: workspace(B, state["np_max"]),
+                 slot_search, nb, B, tree)
+    n_slots, trial, elapsed, overlap_equal = ov.timed(args.slots, args.steps)
+    ms_per_step = elapsed * 1e3 / args.steps
+    qps = B * args.steps / elapsed
+    log(f"batched, {n_slots} in flight: {ms_per_step:.3f} ms/batch, {qps:.0f} QPS (trial by slots: {trial}); "
+        f"overlapped results equal to one batch at a time: {overlap_equal}")
     # Kernels timed with HIP events on the search stream: the data-level scan (HBM-bound:
     # algorithmic bytes = what the reference's schedule reads), and for two-level indexes the
     # meso stage, split into the two-phase scan's summary phase (FP32 vector-ALU bound:
@@ -499,6 +516,7 @@ def main():
         "dtype": "u8 (int32 accumulation)" if u8 else "f32",
         "data": "synthetic (seeded Gaussian mixture; index built in-process)",
         "config": {"workload": workload, "global_batch": B, "parallelism": "single GPU",
+                   "batches_in_flight": n_slots,
                    "scan_block": {"batch": list(plan), "batch64": list(plan64), "one_query": list(plan1)},
                    "meso_two_phase": {"batch": ws.meso_summary, "batch64": ws64.meso_summary,
                                       "one_query": ws1.meso_summary} if tree else None,
@@ -511,6 +529,10 @@ def main():
         "f32_twin": twin,
         "recall_at_4x_max_nprobe": ceiling,
         "parity_vs_oracle": parity,
+        "batches_in_flight": n_slots,
+        "batches_in_flight_trial_qps": trial,
+        "one_batch_in_flight_qps": round(B / solo_ms * 1e3, 1),
+        "overlapped_equal_to_one_at_a_time": overlap_equal,
         "plain_per_query_qps": round(plain_qps, 1),
         "batched_over_plain": round(qps / plain_qps, 2),
         "latency_ms": {"plain_p50": round(float(np.percentile(lat1, 50)), 4),
@@ -521,7 +543,7 @@ def main():
                        "graph_one_query_p99": round(float(np.percentile(lat1g, 99)), 4),
                        "graph_batch64_p50": round(float(np.percentile(lat64g, 50)), 4),
                        f"graph_batch{B}_persistent_queue_median": round(float(np.median(latbg)), 4),
-                       f"batch{B}_mean": round(ms_per_step, 4)},
+                       f"batch{B}_mean_one_in_flight": round(solo_ms, 4)},
         "batch_service_ms": svc,
         "stage_ms": {"preprocess": round(float(stage[0]), 4), "probe": round(float(stage[1]), 4),
                      "meso_scan": round(float(stage[2]), 4), "scan": round(float(stage[3]), 4)},
@@ -810,6 +832,75 @@ def sq8_exhaustive(idx, g, q, gt, s_idist, s_cnt, K, torch, dev, chunk=1 << 19):
             f"pruned_search_top{K}_distances_in_exhaustive_sq8_top{K}": round(float(same_k), 4)}
 
 
+class Overlap:
+    """Batches in flight on streams of their own. Slot j has its own workspace (slot 0 the tuned
+    one; the others made by `make` and given its plan, order and meso mode) and output buffers;
+    `search(ws, start, outs, stream)` enqueues one batch of the queries from `start` on."""
+
+    def __init__(self, torch, dev, ws0, outs0, make, search, nb, B, tree):
+        self.torch, self.dev, self.make, self.search = torch, dev, make, search
+        self.nb, self.B, self.tree = nb, B, tree
+        self.ws, self.outs, self.st = [ws0], [tuple(outs0)], []
+
+    def _grow(self, ns):
+        while len(self.ws) < ns:
+            w = self.make()
+            w.set_plan(*self.ws[0].plan())
+            w.set_order(self.ws[0].order)
+            if self.tree:
+                w.set_meso_summary(self.ws[0].meso_summary)
+            self.ws.append(w)
+            self.outs.append(tuple(self.torch.zeros_like(t) for t in self.outs[0]))
+        while len(self.st) < ns:
+            self.st.append(self.torch.cuda.Stream(device=self.dev))
+
+    def run(self, j, i):
+        self.search(self.ws[j], (i % self.nb) * self.B, self.outs[j], self.st[j].cuda_stream)
+
+    def leg(self, ns, n):
+        """Seconds for n batches with ns in flight (2 ns untimed batches first)."""
+        self._grow(ns)
+        for i in range(2 * ns):
+            self.run(i % ns, i)
+        self.torch.cuda.synchronize()
+        t = time.perf_counter()
+        for i in range(n):
+            self.run(i % ns, i)
+        self.torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    def _valid(self, outs):
+        """Copies of a slot's outputs with the entries past each query's result count zeroed (the
+        counts are the last buffer)."""
+        torch = self.torch
+        c = outs[-1].long()
+        keep = torch.arange(outs[0].shape[1], device=c.device)[None, :] < c[:, None]
+        return [torch.where(keep, t, torch.zeros_like(t)) for t in outs[:-1]] + [c.clone()]
+
+    def timed(self, slots, steps):
+        """(slots used, trial QPS by slot count or None, seconds for `steps` batches, whether each
+        slot's last batch equals the same queries searched alone)."""
+        ws0 = self.ws[0]
+        ws0.enable_timing(False)  # no event records or byte counters between the overlapped batches
+        trial = None
+        if slots <= 0:
+            trial = {ns: round(self.B * 12 / min(self.leg(ns, 12) for _ in range(2)), 1) for ns in (1, 2, 3, 4)}
+            slots = max(trial, key=trial.get)
+        elapsed = self.leg(slots, steps)
+        equal = True
+        for i in range(max(0, steps - slots), steps):
+            got = self._valid(self.outs[i % slots])
+            self.run(0, i)
+            self.torch.cuda.synchronize()
+            equal = equal and all(bool(self.torch.equal(a, b)) for a, b in zip(got, self._valid(self.outs[0])))
+        ws0.enable_timing(True)
+        for w in self.ws[1:]:
+            w.close()
+        self.ws = self.ws[:1]
+        self.outs = self.outs[:1]
+        return slots, trial, elapsed, equal
+
+
 def f32_twin_leg(args, cfg, idx, x, q, gt, nprobe_q, torch, dev, _capi, builder, sp):
     """IndexPDXIVFTree (F32) over the clustering of the SQ8 index (same lists, meso level, order of
     members and row ids; rows stored as the normalised + rotated floats): recall@10, nprobe scaled
@@ -888,15 +979,26 @@ def f32_twin_leg(args, cfg, idx, x, q, gt, nprobe_q, torch, dev, _capi, builder,
         stage += st["ws"].stage_times()
         sbytes += st["ws"].stage_bytes()
     torch.cuda.synchronize()
-    el = time.perf_counter() - t
-    qps = B * args.steps / el
+    solo_ms = (time.perf_counter() - t) * 1e3 / args.steps
     stage /= args.steps
     sbytes /= args.steps
     gbs = sbytes[1] / (stage[3] * 1e-3) / 1e9
-    log(f"f32 twin: {qps:.0f} QPS at recall@10 {r10:.4f} (nprobe x{scale}); stage ms {stage.round(4)}; data scan "
-        f"{gbs:.0f} GB/s; parity vs oracle on {B} queries: {par_ok}")
+
+    def slot_search(w, s0, o, stream):
+        w.search(qd[s0:s0 + B].data_ptr(), B, 0, st["npd"][s0:s0 + B].data_ptr(), K, st["npm"], K, o[0].data_ptr(),
+                 o[1].data_ptr(), 0, o[2].data_ptr(), 0, stream=stream, k_all=K)
+
+    ov = Overlap(torch, dev, st["ws"], (ids, dd, cnt), lambda: _capi.Workspace(g, B, K, st["npm"]), slot_search,
+                 nb, B, True)
+    n_slots, trial, el, equal = ov.timed(args.slots, args.steps)
+    qps = B * args.steps / el
+    log(f"f32 twin: {qps:.0f} QPS with {n_slots} batches in flight ({B / solo_ms * 1e3:.0f} one at a time) at "
+        f"recall@10 {r10:.4f} (nprobe x{scale}); stage ms {stage.round(4)}; data scan {gbs:.0f} GB/s; parity vs "
+        f"oracle on {B} queries: {par_ok}; overlapped equal to one at a time: {equal}")
     out = {"path": "IndexPDXIVFTree (F32) over the SQ8 index's clustering", "qps": round(qps, 1),
-           "ms_per_step": round(el * 1e3 / args.steps, 4), "recall_at_10": round(float(r10), 4),
+           "ms_per_step": round(el * 1e3 / args.steps, 4), "batches_in_flight": n_slots,
+           "batches_in_flight_trial_qps": trial, "one_batch_in_flight_qps": round(B / solo_ms * 1e3, 1),
+           "overlapped_equal_to_one_at_a_time": equal, "recall_at_10": round(float(r10), 4),
            "nprobe": "U[%d,%d] x %d (capped at %d lists)" % (cfg["nprobe_range"] + (scale, nlist))
            if cfg.get("nprobe_range") else str(int(arr[0])),
            "recall_gate_met": bool(r10 >= args.target_recall), "scan_block": list(plan),
