@@ -171,7 +171,10 @@ pdxg_status pdxg_workspace_set_order(pdxg_workspace* ws, int mode);
 int pdxg_workspace_order(const pdxg_workspace* ws);
 
 /* Asynchronous batched search on `stream` (a hipStream_t, NULL = default stream) with device
- * buffers. The workspace must not be shared by calls that overlap in time. */
+ * buffers. The workspace must not be shared by calls that overlap in time. Calls with workspaces
+ * and outputs of their own may overlap on different streams over one index (the index is only
+ * read): throughput callers keep several batches in flight that way, so that one batch's
+ * memory-bound data-level scan runs beside another's compute-bound meso phase. */
 pdxg_status pdxg_search_device(pdxg_index* index, pdxg_workspace* ws, const pdxg_search_args* args,
                                void* stream);
 
