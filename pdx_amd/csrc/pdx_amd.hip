@@ -628,10 +628,10 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     const dim3 gp((unsigned)((pairs + kListPer - 1) / kListPer));
     require(L.nc <= kListHistMax, "too many meso clusters for the visit-list histogram", PDXG_ERR_UNSUPPORTED);
     hipLaunchKernelGGL(k_meso_count, gp, dim3(kListThreads), L.nc * 4, st, m);
-    hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(1024), 0, st, m);
+    hipLaunchKernelGGL(k_meso_offsets, dim3(1), dim3(kScanThreads), 0, st, m);
     hipLaunchKernelGGL(k_meso_scatter, gp, dim3(kListThreads), L.nc * 4, st, m);
     hipLaunchKernelGGL(k_meso_soff, dim3(nq), dim3(kWave), 0, st, m);
-    hipLaunchKernelGGL(k_meso_qbase, dim3(1), dim3(1024), 0, st, m);
+    hipLaunchKernelGGL(k_meso_qbase, dim3(1), dim3(kScanThreads), 0, st, m);
     // work items (counted on the device): sum over clusters of ceil(entries / kMesoQ) x chunks,
     // where a cluster's entries are at most nq and all entries at most `pairs`; the grid covers
     // this bound, one item per wave

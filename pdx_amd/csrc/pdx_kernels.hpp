@@ -146,9 +146,10 @@ __global__ void k_setup(SetupArgs a) {
 // Longest-first launch order of the per-query scans: queries sorted by descending visit count
 // (counting sort, counts above kLptBuckets - 1 share the top bucket; order within a bucket is
 // unspecified). Workgroups are dispatched in launch order, so the longest walks start first and
-// the short ones fill in behind them. One workgroup of kLptThreads threads.
+// the short ones fill in behind them. One workgroup of kLptThreads threads: small, so that it finds
+// room on a CU at once while kernels of other batches in flight hold the rest.
 constexpr uint32_t kLptBuckets = 4096;
-constexpr uint32_t kLptThreads = 1024;
+constexpr uint32_t kLptThreads = 256;
 __global__ __launch_bounds__(kLptThreads) void k_lpt_order(const uint32_t* __restrict__ key, uint32_t nq,
                                                            uint32_t* __restrict__ perm) {
     __shared__ uint32_t cur[kLptBuckets];

@@ -79,18 +79,23 @@ __global__ __launch_bounds__(kListThreads) void k_meso_count(MesoArgs a) {
         if (hist[i]) atomicAdd(&a.cnt[i], hist[i]);
 }
 
-// Entry and work-item offsets per cluster (one workgroup of 1024).
-__global__ __launch_bounds__(1024) void k_meso_offsets(MesoArgs a) {
-    __shared__ uint32_t red[1024 / kWave + 1];
+// Single-workgroup kernels of the meso phase run with kScanThreads threads: a small workgroup finds
+// room on a CU at once while kernels of other batches in flight hold the rest (a 1024-thread one
+// waited for a whole CU's worth of free wave slots).
+constexpr uint32_t kScanThreads = 256;
+
+// Entry and work-item offsets per cluster (one workgroup).
+__global__ __launch_bounds__(kScanThreads) void k_meso_offsets(MesoArgs a) {
+    __shared__ uint32_t red[kScanThreads / kWave + 1];
     const uint32_t nc = a.L.nc;
     uint32_t lbase = 0, cbase = 0;
-    for (uint32_t m0 = 0; m0 < nc; m0 += 1024) {
+    for (uint32_t m0 = 0; m0 < nc; m0 += kScanThreads) {
         const uint32_t m = m0 + threadIdx.x;
         const uint32_t c = m < nc ? a.cnt[m] : 0u;
         uint32_t lt, ct;
-        const uint32_t lx = block_excl_sum<1024>(c, red, &lt);
+        const uint32_t lx = block_excl_sum<kScanThreads>(c, red, &lt);
         const uint32_t nch = m < nc ? max(1u, (a.L.n_used[m] + kWave - 1) / kWave) : 0u;
-        const uint32_t cx = block_excl_sum<1024>((c + kMesoQ - 1) / kMesoQ * nch, red, &ct);
+        const uint32_t cx = block_excl_sum<kScanThreads>((c + kMesoQ - 1) / kMesoQ * nch, red, &ct);
         if (m < nc) {
             a.lstart[m] = lbase + lx;
             a.cstart[m] = cbase + cx;
@@ -174,16 +179,26 @@ __global__ __launch_bounds__(kWave) void k_meso_soff(MesoArgs a) {
 }
 
 // Region starts: exclusive running sum of the per-query totals in qbase[0 .. nq), in place, and
-// the grand total in qbase[nq] (one workgroup of 1024).
-__global__ __launch_bounds__(1024) void k_meso_qbase(MesoArgs a) {
-    __shared__ uint32_t red[1024 / kWave + 1];
+// the grand total in qbase[nq] (one workgroup; each thread takes 4 consecutive queries of a chunk).
+__global__ __launch_bounds__(kScanThreads) void k_meso_qbase(MesoArgs a) {
+    __shared__ uint32_t red[kScanThreads / kWave + 1];
+    constexpr uint32_t kPer = 4;
     unsigned long long base = 0;
-    for (uint32_t q0 = 0; q0 < a.nq; q0 += 1024) {
-        const uint32_t q = q0 + threadIdx.x;
-        const uint32_t x = q < a.nq ? static_cast<uint32_t>(a.qbase[q]) : 0u;  // one query's total < 2^32
+    for (uint32_t q0 = 0; q0 < a.nq; q0 += kScanThreads * kPer) {
+        const uint32_t qt = q0 + threadIdx.x * kPer;
+        uint32_t x[kPer], sum = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            x[u] = qt + u < a.nq ? static_cast<uint32_t>(a.qbase[qt + u]) : 0u;  // one query's total < 2^32
+            sum += x[u];
+        }
         uint32_t tot;
-        const uint32_t ex = block_excl_sum<1024>(x, red, &tot);
-        if (q < a.nq) a.qbase[q] = base + ex;
+        unsigned long long ex = base + block_excl_sum<kScanThreads>(sum, red, &tot);
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            if (qt + u < a.nq) a.qbase[qt + u] = ex;
+            ex += x[u];
+        }
         base += tot;
     }
     if (threadIdx.x == 0) a.qbase[a.nq] = base;
