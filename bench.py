@@ -211,6 +211,9 @@ def main():
         log(f"copy calibration: {args.copy_calib_mib} MiB device-to-device copy")
         del a, bcopy
 
+    copy_gbs = device_copy_gbs(torch, dev)
+    log(f"1 GiB device-to-device copy: {copy_gbs:.0f} GB/s (read + write)")
+
     B, K, NQ = cfg["batch"], cfg["k"], cfg["nq"]
     rng = np.random.default_rng(args.seed + 1)
     if cfg.get("nprobe_choices"):
@@ -551,9 +554,32 @@ def main():
         "kernels": roofs,
         "serving": serving,
         "cpu_baseline": cpu,
+        "device_copy_gbs": {"value": round(copy_gbs, 1), "what": "1 GiB device-to-device copy (torch), read + write "
+                                                              "bytes per second on this GPU: the rate a plain "
+                                                              "streaming kernel reaches, for scale against the "
+                                                              "scans' HBM traffic rates"},
         "build_s": round(build_s, 1),
     }
     print(json.dumps(line), flush=True)
+
+
+def device_copy_gbs(torch, dev, mib=1024, reps=5):
+    """Read + write GB/s of a plain device-to-device copy (HIP events around `reps` copies)."""
+    a = torch.empty(mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * (mib << 20) / (ms * 1e-3) / 1e9
 
 
 def bench_dense(args, cfg, torch, _capi, builder, dev):
