@@ -630,13 +630,48 @@ struct Scanner {
         __syncthreads();
     }
 
+    // Slot of element t of this thread's vertical tile at i0 (a tile covers VT * BLOCK slots).
+    // F32: slot i0 + t*BLOCK + tid. U8 (VT a multiple of 4): four consecutive slots per thread in
+    // each 4*BLOCK-slot part, so one 16-byte load reads their words of a dim group.
+    template <uint32_t VT>
+    static __device__ __forceinline__ uint32_t tile_slot(uint32_t i0, uint32_t t) {
+        if constexpr (U8 && VT % 4 == 0) return i0 + (t >> 2) * 4 * BLOCK + 4 * threadIdx.x + (t & 3);
+        else return i0 + t * BLOCK + threadIdx.x;
+    }
+
+    // Stores element t of a vertical tile's results into dist[] (slots >= n are dropped): U8
+    // quads that lie inside the cluster go out as one 16-byte LDS store (o_dist is 16-byte aligned).
+    template <uint32_t VT>
+    __device__ __forceinline__ void store_tile(const dist_t (&acc)[VT], uint32_t i0) const {
+        if constexpr (U8 && VT % 4 == 0) {
+#pragma unroll
+            for (uint32_t r = 0; r < VT / 4; ++r) {
+                const uint32_t s = tile_slot<VT>(i0, 4 * r);
+                if (s + 3 < n) {
+                    *reinterpret_cast<uint4*>(dist + s) =
+                        make_uint4(acc[4 * r], acc[4 * r + 1], acc[4 * r + 2], acc[4 * r + 3]);
+                } else {
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e)
+                        if (s + e < n) dist[s + e] = acc[4 * r + e];
+                }
+            }
+        } else {
+#pragma unroll
+            for (uint32_t t = 0; t < VT; ++t) {
+                const uint32_t i = tile_slot<VT>(i0, t);
+                if (i < n) dist[i] = acc[t];
+            }
+        }
+    }
+
     // Vertical (scalar_computers.hpp:19-44 / 77-121) over dims [da, db) for the VT slots
-    // i0 + t*BLOCK + tid of this thread; slots >= n are clamped to a valid slot and ignored.
+    // tile_slot(i0, t) of this thread; slots >= n are clamped to a valid slot and ignored.
     template <uint32_t VT = kVT>
     __device__ __forceinline__ void vertical_tile(dist_t (&acc)[VT], uint32_t i0, uint32_t da, uint32_t db) const {
         uint32_t idx[VT];
 #pragma unroll
-        for (uint32_t t = 0; t < VT; ++t) idx[t] = min(i0 + t * BLOCK + threadIdx.x, n - 1);
+        for (uint32_t t = 0; t < VT; ++t) idx[t] = min(tile_slot<VT>(i0, t), n - 1);
         if constexpr (!U8) {
             const float* data = reinterpret_cast<const float*>(base);
             uint32_t j = da;
@@ -682,45 +717,59 @@ struct Scanner {
                 }
             }
         } else {
+            static_assert(VT % 4 == 0, "U8 vertical tiles hold whole quads of slots");
+            constexpr uint32_t VQ = VT / 4;
+            // first slot of each of this thread's quads; a quad holding a valid slot lies inside
+            // the dim group's row (device strides are multiples of 32 slots, rows 256-byte
+            // aligned), the others are clamped to the last quad and ignored
+            uint32_t q4[VQ];
+#pragma unroll
+            for (uint32_t r = 0; r < VQ; ++r) q4[r] = min(tile_slot<VT>(i0, 4 * r), (n - 1) & ~3u);
+            auto word4 = [&](uint32_t j, uint32_t r) {
+                return *reinterpret_cast<const uint4*>(base + (size_t)j * S + 4 * q4[r]);
+            };
+            auto add4 = [&](uint32_t r, const uint4& x, uint32_t qv, uint32_t qs) {
+                acc[4 * r] += sqdist4(qv, x.x, qs);
+                acc[4 * r + 1] += sqdist4(qv, x.y, qs);
+                acc[4 * r + 2] += sqdist4(qv, x.z, qs);
+                acc[4 * r + 3] += sqdist4(qv, x.w, qs);
+            };
             uint32_t j = da;
             // 32 dims (8 words) x VT vectors of loads in flight per round trip (VT <= 4: wider
             // tiles take 16-dim rounds)
             for (; VT <= 4 && j + 32 <= db; j += 32) {
-                uint32_t x[8][VT];
+                uint4 x[8][VQ];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t)
-                        x[u][t] = *reinterpret_cast<const uint32_t*>(base + (size_t)(j + 4 * u) * S + 4 * idx[t]);
+                    for (uint32_t r = 0; r < VQ; ++r) x[u][r] = word4(j + 4 * u, r);
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u) {
                     const uint32_t w = (j >> 2) + u;
                     const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
+                    for (uint32_t r = 0; r < VQ; ++r) add4(r, x[u][r], qv, qs);
                 }
             }
             for (; j + 16 <= db; j += 16) {
-                uint32_t x[4][VT];
+                uint4 x[4][VQ];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t)
-                        x[u][t] = *reinterpret_cast<const uint32_t*>(base + (size_t)(j + 4 * u) * S + 4 * idx[t]);
+                    for (uint32_t r = 0; r < VQ; ++r) x[u][r] = word4(j + 4 * u, r);
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     const uint32_t w = (j >> 2) + u;
                     const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t) acc[t] += sqdist4(qv, x[u][t], qs);
+                    for (uint32_t r = 0; r < VQ; ++r) add4(r, x[u][r], qv, qs);
                 }
             }
             for (; j + 4 <= db; j += 4) {
                 const uint32_t w = j >> 2;
                 const uint32_t qv = qw[w], qs = sumsq4(qv);
 #pragma unroll
-                for (uint32_t t = 0; t < VT; ++t)
-                    acc[t] += sqdist4(qv, *reinterpret_cast<const uint32_t*>(base + (size_t)j * S + 4 * idx[t]), qs);
+                for (uint32_t r = 0; r < VQ; ++r) add4(r, word4(j, r), qv, qs);
             }
             if (j < db) {  // tail group of rem < 4 dims, stored as rem bytes per vector
                 const uint32_t rem = db - j;
@@ -960,11 +1009,7 @@ struct Scanner {
 #pragma unroll
             for (uint32_t t = 0; t < kVT; ++t) acc[t] = dist_t(0);
             vertical_tile(acc, i0, 0, v);
-#pragma unroll
-            for (uint32_t t = 0; t < kVT; ++t) {
-                uint32_t i = i0 + t * BLOCK + threadIdx.x;
-                if (i < n) dist[i] = acc[t];
-            }
+            store_tile<kVT>(acc, i0);
         }
         __syncthreads();
         if (h) {
@@ -1027,16 +1072,9 @@ struct Scanner {
             for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
                 dist_t acc[kVT];
 #pragma unroll
-                for (uint32_t t = 0; t < kVT; ++t) {
-                    uint32_t i = min(i0 + t * BLOCK + threadIdx.x, n - 1);
-                    acc[t] = dist[i];
-                }
+                for (uint32_t t = 0; t < kVT; ++t) acc[t] = dist[min(tile_slot<kVT>(i0, t), n - 1)];
                 vertical_tile(acc, i0, 0, v);
-#pragma unroll
-                for (uint32_t t = 0; t < kVT; ++t) {
-                    uint32_t i = i0 + t * BLOCK + threadIdx.x;
-                    if (i < n) dist[i] = acc[t];
-                }
+                store_tile<kVT>(acc, i0);
             }
             bytes += (unsigned long long)n * v * (U8 ? 1 : 4);
         } else {
@@ -1055,19 +1093,36 @@ struct Scanner {
     }
 
     // searcher.hpp:376-435. Returns the dimension index reached.
-    // One tile of a warmup step: the vertical dims [cur, end) of slots i0 + u * BLOCK + tid
-    // (u < VT); returns the wave's count of dist >= tau over the tile.
+    // One tile of a warmup step: the vertical dims [cur, end) of slots tile_slot(i0, u) (u < VT);
+    // returns the wave's count of dist >= tau over the tile.
     template <uint32_t VT>
     __device__ __forceinline__ uint32_t warmup_tile(uint32_t i0, uint32_t cur, uint32_t end, dist_t tau) {
         dist_t acc[VT];
+        if constexpr (U8) {
 #pragma unroll
-        for (uint32_t u = 0; u < VT; ++u) acc[u] = dist[min(i0 + u * BLOCK + threadIdx.x, n - 1)];
+            for (uint32_t r = 0; r < VT / 4; ++r) {
+                const uint32_t s = tile_slot<VT>(i0, 4 * r);
+                if (s + 3 < n) {
+                    const uint4 x = *reinterpret_cast<const uint4*>(dist + s);
+                    acc[4 * r] = x.x;
+                    acc[4 * r + 1] = x.y;
+                    acc[4 * r + 2] = x.z;
+                    acc[4 * r + 3] = x.w;
+                } else {
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e) acc[4 * r + e] = dist[min(s + e, n - 1)];
+                }
+            }
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < VT; ++u) acc[u] = dist[min(tile_slot<VT>(i0, u), n - 1)];
+        }
         vertical_tile<VT>(acc, i0, cur, end);
+        store_tile<VT>(acc, i0);
         uint32_t cnt = 0;
 #pragma unroll
         for (uint32_t u = 0; u < VT; ++u) {
-            uint32_t i = i0 + u * BLOCK + threadIdx.x;
-            if (i < n) dist[i] = acc[u];
+            const uint32_t i = tile_slot<VT>(i0, u);
             cnt += static_cast<uint32_t>(__popcll(__ballot(i < n && acc[u] >= tau)));
         }
         return cnt;
