@@ -630,37 +630,48 @@ struct Scanner {
         __syncthreads();
     }
 
-    // Slot of element t of this thread's vertical tile at i0 (a tile covers VT * BLOCK slots).
-    // F32: slot i0 + t*BLOCK + tid. U8 (VT a multiple of 4): four consecutive slots per thread in
-    // each 4*BLOCK-slot part, so one 16-byte load reads their words of a dim group.
+    // Slot of element t of this thread's vertical tile at i0: a tile covers VT * BLOCK slots
+    // (VT a multiple of 4), four consecutive slots per thread in each 4*BLOCK-slot part, so one
+    // 16-byte load reads their values (F32) or words (U8) of a dim (group).
     template <uint32_t VT>
     static __device__ __forceinline__ uint32_t tile_slot(uint32_t i0, uint32_t t) {
-        if constexpr (U8 && VT % 4 == 0) return i0 + (t >> 2) * 4 * BLOCK + 4 * threadIdx.x + (t & 3);
-        else return i0 + t * BLOCK + threadIdx.x;
+        static_assert(VT % 4 == 0, "vertical tiles hold whole quads of slots");
+        return i0 + (t >> 2) * 4 * BLOCK + 4 * threadIdx.x + (t & 3);
     }
 
-    // Stores element t of a vertical tile's results into dist[] (slots >= n are dropped): U8
-    // quads that lie inside the cluster go out as one 16-byte LDS store (o_dist is 16-byte aligned).
+    // A vertical tile's distances from / into dist[] (slots >= n are clamped on the way in and
+    // dropped on the way out): quads inside the cluster move as one 16-byte LDS access (o_dist is
+    // 16-byte aligned).
+    template <uint32_t VT>
+    __device__ __forceinline__ void load_tile(dist_t (&acc)[VT], uint32_t i0) const {
+#pragma unroll
+        for (uint32_t r = 0; r < VT / 4; ++r) {
+            const uint32_t s = tile_slot<VT>(i0, 4 * r);
+            if (s + 3 < n) {
+                const uint4 x = *reinterpret_cast<const uint4*>(dist + s);
+                acc[4 * r] = __builtin_bit_cast(dist_t, x.x);
+                acc[4 * r + 1] = __builtin_bit_cast(dist_t, x.y);
+                acc[4 * r + 2] = __builtin_bit_cast(dist_t, x.z);
+                acc[4 * r + 3] = __builtin_bit_cast(dist_t, x.w);
+            } else {
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) acc[4 * r + e] = dist[min(s + e, n - 1)];
+            }
+        }
+    }
     template <uint32_t VT>
     __device__ __forceinline__ void store_tile(const dist_t (&acc)[VT], uint32_t i0) const {
-        if constexpr (U8 && VT % 4 == 0) {
 #pragma unroll
-            for (uint32_t r = 0; r < VT / 4; ++r) {
-                const uint32_t s = tile_slot<VT>(i0, 4 * r);
-                if (s + 3 < n) {
-                    *reinterpret_cast<uint4*>(dist + s) =
-                        make_uint4(acc[4 * r], acc[4 * r + 1], acc[4 * r + 2], acc[4 * r + 3]);
-                } else {
+        for (uint32_t r = 0; r < VT / 4; ++r) {
+            const uint32_t s = tile_slot<VT>(i0, 4 * r);
+            if (s + 3 < n) {
+                *reinterpret_cast<uint4*>(dist + s) = make_uint4(
+                    __builtin_bit_cast(uint32_t, acc[4 * r]), __builtin_bit_cast(uint32_t, acc[4 * r + 1]),
+                    __builtin_bit_cast(uint32_t, acc[4 * r + 2]), __builtin_bit_cast(uint32_t, acc[4 * r + 3]));
+            } else {
 #pragma unroll
-                    for (uint32_t e = 0; e < 4; ++e)
-                        if (s + e < n) dist[s + e] = acc[4 * r + e];
-                }
-            }
-        } else {
-#pragma unroll
-            for (uint32_t t = 0; t < VT; ++t) {
-                const uint32_t i = tile_slot<VT>(i0, t);
-                if (i < n) dist[i] = acc[t];
+                for (uint32_t e = 0; e < 4; ++e)
+                    if (s + e < n) dist[s + e] = acc[4 * r + e];
             }
         }
     }
@@ -669,62 +680,66 @@ struct Scanner {
     // tile_slot(i0, t) of this thread; slots >= n are clamped to a valid slot and ignored.
     template <uint32_t VT = kVT>
     __device__ __forceinline__ void vertical_tile(dist_t (&acc)[VT], uint32_t i0, uint32_t da, uint32_t db) const {
-        uint32_t idx[VT];
+        constexpr uint32_t VQ = VT / 4;
+        // first slot of each of this thread's quads; a quad holding a valid slot lies inside its
+        // dim's (group's) row (device strides are multiples of 32 slots, rows 256-byte aligned),
+        // the others are clamped to the last quad and ignored
+        uint32_t q4[VQ];
 #pragma unroll
-        for (uint32_t t = 0; t < VT; ++t) idx[t] = min(tile_slot<VT>(i0, t), n - 1);
+        for (uint32_t r = 0; r < VQ; ++r) q4[r] = min(tile_slot<VT>(i0, 4 * r), (n - 1) & ~3u);
         if constexpr (!U8) {
             const float* data = reinterpret_cast<const float*>(base);
+            auto val4 = [&](uint32_t j, uint32_t r) {
+                return *reinterpret_cast<const float4*>(data + (size_t)j * S + q4[r]);
+            };
+            // each slot's chain runs over the dims in order
+            auto fma4 = [&](uint32_t r, const float4& x, float qj) {
+                float df = qj - x.x;
+                acc[4 * r] = __builtin_fmaf(df, df, acc[4 * r]);
+                df = qj - x.y;
+                acc[4 * r + 1] = __builtin_fmaf(df, df, acc[4 * r + 1]);
+                df = qj - x.z;
+                acc[4 * r + 2] = __builtin_fmaf(df, df, acc[4 * r + 2]);
+                df = qj - x.w;
+                acc[4 * r + 3] = __builtin_fmaf(df, df, acc[4 * r + 3]);
+            };
             uint32_t j = da;
             // 8 dims x VT vectors of loads in flight per round trip
             for (; j + 8 <= db; j += 8) {
-                float x[8][VT];
+                float4 x[8][VQ];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
+                    for (uint32_t r = 0; r < VQ; ++r) x[u][r] = val4(j + u, r);
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u) {
                     const float qj = qf[j + u];
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t) {
-                        float df = qj - x[u][t];
-                        acc[t] = __builtin_fmaf(df, df, acc[t]);
-                    }
+                    for (uint32_t r = 0; r < VQ; ++r) fma4(r, x[u][r], qj);
                 }
             }
             for (; j + 4 <= db; j += 4) {
-                float x[4][VT];
+                float4 x[4][VQ];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t) x[u][t] = data[(size_t)(j + u) * S + idx[t]];
+                    for (uint32_t r = 0; r < VQ; ++r) x[u][r] = val4(j + u, r);
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     const float qj = qf[j + u];
 #pragma unroll
-                    for (uint32_t t = 0; t < VT; ++t) {
-                        float df = qj - x[u][t];
-                        acc[t] = __builtin_fmaf(df, df, acc[t]);
-                    }
+                    for (uint32_t r = 0; r < VQ; ++r) fma4(r, x[u][r], qj);
                 }
             }
             for (; j < db; ++j) {
                 const float qj = qf[j];
 #pragma unroll
-                for (uint32_t t = 0; t < VT; ++t) {
-                    float df = qj - data[(size_t)j * S + idx[t]];
-                    acc[t] = __builtin_fmaf(df, df, acc[t]);
-                }
+                for (uint32_t r = 0; r < VQ; ++r) fma4(r, val4(j, r), qj);
             }
         } else {
-            static_assert(VT % 4 == 0, "U8 vertical tiles hold whole quads of slots");
-            constexpr uint32_t VQ = VT / 4;
-            // first slot of each of this thread's quads; a quad holding a valid slot lies inside
-            // the dim group's row (device strides are multiples of 32 slots, rows 256-byte
-            // aligned), the others are clamped to the last quad and ignored
-            uint32_t q4[VQ];
+            uint32_t idx[VT];
 #pragma unroll
-            for (uint32_t r = 0; r < VQ; ++r) q4[r] = min(tile_slot<VT>(i0, 4 * r), (n - 1) & ~3u);
+            for (uint32_t t = 0; t < VT; ++t) idx[t] = min(tile_slot<VT>(i0, t), n - 1);
             auto word4 = [&](uint32_t j, uint32_t r) {
                 return *reinterpret_cast<const uint4*>(base + (size_t)j * S + 4 * q4[r]);
             };
@@ -1071,8 +1086,7 @@ struct Scanner {
         if (selection > (1.0f - 0.8f)) {
             for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * kVT) {
                 dist_t acc[kVT];
-#pragma unroll
-                for (uint32_t t = 0; t < kVT; ++t) acc[t] = dist[min(tile_slot<kVT>(i0, t), n - 1)];
+                load_tile<kVT>(acc, i0);
                 vertical_tile(acc, i0, 0, v);
                 store_tile<kVT>(acc, i0);
             }
@@ -1098,25 +1112,7 @@ struct Scanner {
     template <uint32_t VT>
     __device__ __forceinline__ uint32_t warmup_tile(uint32_t i0, uint32_t cur, uint32_t end, dist_t tau) {
         dist_t acc[VT];
-        if constexpr (U8) {
-#pragma unroll
-            for (uint32_t r = 0; r < VT / 4; ++r) {
-                const uint32_t s = tile_slot<VT>(i0, 4 * r);
-                if (s + 3 < n) {
-                    const uint4 x = *reinterpret_cast<const uint4*>(dist + s);
-                    acc[4 * r] = x.x;
-                    acc[4 * r + 1] = x.y;
-                    acc[4 * r + 2] = x.z;
-                    acc[4 * r + 3] = x.w;
-                } else {
-#pragma unroll
-                    for (uint32_t e = 0; e < 4; ++e) acc[4 * r + e] = dist[min(s + e, n - 1)];
-                }
-            }
-        } else {
-#pragma unroll
-            for (uint32_t u = 0; u < VT; ++u) acc[u] = dist[min(tile_slot<VT>(i0, u), n - 1)];
-        }
+        load_tile<VT>(acc, i0);
         vertical_tile<VT>(acc, i0, cur, end);
         store_tile<VT>(acc, i0);
         uint32_t cnt = 0;
