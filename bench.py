@@ -380,7 +380,7 @@ def main():
         flops = pairs * (3 * cfg["d"] + 15 * (idx.l1.h // 64))
         tf = flops / (meso[0] * 1e-3) / 1e12
         roofs["meso_phase1"] = {"bound": "valu",
-                                "kernel": "k_meso_summary + its visit-list kernels (two-phase meso scan, phase 1)",
+                                "kernel": "k_meso_summary + k_meso_tails + their visit-list kernels (two-phase meso scan, phase 1)",
                                 "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                 "frac": round(tf / FP32_PEAK_TFLOPS, 4), "ms": round(float(meso[0]), 4),
                                 "pairs_per_launch": int(pairs), "flops_per_launch": int(flops)}

@@ -343,13 +343,14 @@ struct pdxg_workspace {
     // optional: the batch's per-query nprobe on the host (a server's staging), so that the batch's
     // summaries are checked against summ_cap by each query's own bound instead of `region`
     const uint32_t* host_nprobe = nullptr;
-    DevBuf<uint32_t> m_cnt, m_cur, m_lstart, m_cstart, m_list, m_soff;
+    DevBuf<uint32_t> m_cnt, m_cur, m_lstart, m_cstart, m_tstart, m_list, m_soff;
     DevBuf<unsigned long long> m_qbase;
     DevBuf<float> summ;
     size_t bytes() const {
         return perm.bytes() + queue.bytes() + qnorm.bytes() + qrot.bytes() + probe.bytes() + qu8.bytes() + k1.bytes() + nv1.bytes() + k0.bytes() +
                nv0.bytes() + order.bytes() + ids0.bytes() + cnt0.bytes() + m_cnt.bytes() + m_cur.bytes() +
-               m_lstart.bytes() + m_cstart.bytes() + m_list.bytes() + m_soff.bytes() + m_qbase.bytes() + summ.bytes();
+               m_lstart.bytes() + m_cstart.bytes() + m_tstart.bytes() + m_list.bytes() + m_soff.bytes() +
+               m_qbase.bytes() + summ.bytes();
     }
 };
 
@@ -447,7 +448,7 @@ size_t workspace_bytes_for(const pdxg_index* ix, uint32_t nq, uint32_t max_k, ui
     b += (size_t)nq * ws.order_stride * 4; // order
     b += (size_t)nq * ws.k0_max * 4 + (size_t)nq * 4;
     if (ws.summ_ok) {
-        b += 4 * ((size_t)ix->l0.nc + 1) * 4;        // counts, cursors, entry / item offsets
+        b += 5 * ((size_t)ix->l0.nc + 1) * 4;        // counts, cursors, entry / item / tail item offsets
         b += (size_t)nq * ws.nv0_max * 4;            // entries
         b += (size_t)nq * ws.order_stride * 4;       // summary offsets
         b += (size_t)(nq + 1) * 8;                   // region starts
@@ -480,6 +481,7 @@ void alloc_workspace(const pdxg_index* ix, pdxg_workspace& ws) {
         ws.m_cur.alloc(n0 + 1);
         ws.m_lstart.alloc(n0 + 1);
         ws.m_cstart.alloc(n0 + 1);
+        ws.m_tstart.alloc(n0 + 1);
         ws.m_list.alloc(nq * ws.nv0_max);
         ws.m_soff.alloc(nq * ws.order_stride);
         ws.m_qbase.alloc(nq + 1);
@@ -614,6 +616,7 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     m.cur = ws.m_cur.p;
     m.lstart = ws.m_lstart.p;
     m.cstart = ws.m_cstart.p;
+    m.tstart = ws.m_tstart.p;
     m.list = ws.m_list.p;
     m.soff = ws.m_soff.p;
     m.summ = ws.summ.p;
@@ -647,6 +650,18 @@ void launch_meso_summary(const pdxg_index* ix, pdxg_workspace& ws, uint32_t nq, 
     require(8 * wpx * kMesoWaves * kWave < (1ull << 32), "too many meso work items for one dispatch",
             PDXG_ERR_UNSUPPORTED);
     hipLaunchKernelGGL(k_meso_summary, dim3((uint32_t)(8 * wpx)), dim3(kMesoWaves * kWave), 0, st, m);
+    // packed tail items (counted on the device): at most one per 8-entry group of the clusters
+    // whose last chunk holds at most 32 vectors, each run as kTailParts waves
+    size_t packed = 0;
+    for (uint32_t c = 0; c < L.nc; ++c) packed += meso_tail_groups(L.n_used[c]) > 1 ? 1 : 0;
+    if (packed) {
+        const size_t titems = kTailParts * std::min((pairs + kMesoQ - 1) / kMesoQ + packed,
+                                                    ((size_t)nq + kMesoQ - 1) / kMesoQ * packed);
+        const size_t tpx = ((titems + 7) / 8 + kMesoWaves - 1) / kMesoWaves;
+        require(8 * tpx * kMesoWaves * kWave < (1ull << 32), "too many meso tail items for one dispatch",
+                PDXG_ERR_UNSUPPORTED);
+        hipLaunchKernelGGL(k_meso_tails, dim3((uint32_t)(8 * tpx)), dim3(kMesoWaves * kWave), 0, st, m);
+    }
     HIP_CHECK(hipGetLastError());
 }
 

@@ -25,6 +25,17 @@ namespace pdxg {
 constexpr uint32_t kMesoQ = 8;      // queries per work item (one wave)
 constexpr uint32_t kMesoWaves = 4;  // work items per workgroup
 
+// Groups of kMesoQ entries that share one wave on the last 64-vector chunk of a cluster of n
+// vectors: 4 when that chunk holds at most 16 vectors, 2 when at most 32 (a packed tail item, lanes
+// split between the groups), else 1 (the chunk is an ordinary work item).
+__host__ __device__ constexpr uint32_t meso_tail_groups(uint32_t n) {
+    return n == 0 ? 1u : (n - (n - 1) / 64 * 64 <= 16 ? 4u : (n - (n - 1) / 64 * 64 <= 32 ? 2u : 1u));
+}
+static_assert(kWave == 64, "meso_tail_groups assumes 64-lane waves");
+static_assert(meso_tail_groups(64) == 1 && meso_tail_groups(65) == 4 && meso_tail_groups(96) == 2 &&
+                  meso_tail_groups(97) == 1 && meso_tail_groups(16) == 4 && meso_tail_groups(17) == 2,
+              "tail sizes");
+
 struct MesoArgs {
     LevelDev L;                 // the meso level (F32)
     const float* qf;            // rotated queries [nq x d]
@@ -35,7 +46,9 @@ struct MesoArgs {
     uint32_t* cur;              // [nc] scatter cursors
     uint32_t* lstart;           // [nc + 1] first entry of each cluster
     uint32_t* cstart;           // [nc + 1] first work item of each cluster; a cluster's items are
-                                // (8-entry group, 64-vector chunk) pairs, chunk-minor
+                                // (8-entry group, 64-vector chunk) pairs, chunk-minor; a last
+                                // chunk of at most 32 vectors is left to the packed tail items
+    uint32_t* tstart;           // [nc + 1] first packed tail item of each cluster (meso_tail_groups)
     uint32_t* list;             // entries q * stride + rank, grouped by cluster
     uint32_t* soff;             // [nq x stride] first summarised vector of (query, rank) in its region
     unsigned long long* qbase;  // [nq + 1] first summarised vector of each query's region (the
@@ -88,25 +101,31 @@ constexpr uint32_t kScanThreads = 256;
 __global__ __launch_bounds__(kScanThreads) void k_meso_offsets(MesoArgs a) {
     __shared__ uint32_t red[kScanThreads / kWave + 1];
     const uint32_t nc = a.L.nc;
-    uint32_t lbase = 0, cbase = 0;
+    uint32_t lbase = 0, cbase = 0, tbase = 0;
     for (uint32_t m0 = 0; m0 < nc; m0 += kScanThreads) {
         const uint32_t m = m0 + threadIdx.x;
         const uint32_t c = m < nc ? a.cnt[m] : 0u;
-        uint32_t lt, ct;
+        uint32_t lt, ct, tt;
         const uint32_t lx = block_excl_sum<kScanThreads>(c, red, &lt);
-        const uint32_t nch = m < nc ? max(1u, (a.L.n_used[m] + kWave - 1) / kWave) : 0u;
-        const uint32_t cx = block_excl_sum<kScanThreads>((c + kMesoQ - 1) / kMesoQ * nch, red, &ct);
+        const uint32_t n = m < nc ? a.L.n_used[m] : 0u;
+        const uint32_t nch = m < nc ? max(1u, (n + kWave - 1) / kWave) : 0u;
+        const uint32_t tg = meso_tail_groups(n), groups = (c + kMesoQ - 1) / kMesoQ;
+        const uint32_t cx = block_excl_sum<kScanThreads>(groups * (tg > 1 ? nch - 1 : nch), red, &ct);
+        const uint32_t tx = block_excl_sum<kScanThreads>(tg > 1 ? (groups + tg - 1) / tg : 0u, red, &tt);
         if (m < nc) {
             a.lstart[m] = lbase + lx;
             a.cstart[m] = cbase + cx;
+            a.tstart[m] = tbase + tx;
             a.cur[m] = 0;
         }
         lbase += lt;
         cbase += ct;
+        tbase += tt;
     }
     if (threadIdx.x == 0) {
         a.lstart[nc] = lbase;
         a.cstart[nc] = cbase;
+        a.tstart[nc] = tbase;
     }
 }
 
@@ -349,13 +368,15 @@ __device__ __forceinline__ void meso_item(const MesoArgs& a, uint32_t item) {
         else hi = mid;
     }
     const uint32_t m = lo;
-    const uint32_t nch = max(1u, (a.L.n_used[m] + kWave - 1) / kWave);
+    const uint32_t n = a.L.n_used[m];
+    if (n == 0) return;
+    // chunks of this cluster in ordinary items (its last one is a packed tail item's when small;
+    // the cluster has an item here, so at least one chunk remains)
+    const uint32_t nch = (n + kWave - 1) / kWave - (meso_tail_groups(n) > 1 ? 1u : 0u);
     const uint32_t local = item - a.cstart[m];
     const uint32_t e0 = a.lstart[m] + (local / nch) * kMesoQ;
     const uint32_t i_begin = (local % nch) * kWave;
     const uint32_t ne = min(kMesoQ, a.lstart[m + 1] - e0);
-    const uint32_t n = a.L.n_used[m];
-    if (n == 0) return;
     const uint32_t d = a.L.d, S = a.L.stride[m];
     const float* data = reinterpret_cast<const float*>(a.L.data + a.L.data_off[m]);
 
@@ -399,6 +420,165 @@ __global__ __launch_bounds__(256) void k_meso_summary(MesoArgs a) {
     const uint32_t local = blockIdx.x / 8 * kMesoWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t item = blockIdx.x % 8 * per + local;
     if (local < per && item < total) meso_item(a, item);
+}
+
+// Packed tail items. The last chunk of a cluster holding at most 32 (16) vectors is computed for 2
+// (4) groups of kMesoQ entries on one wave: lane l takes vector l mod W of the chunk (W = 32 or 16)
+// for the entries of group l / W, so no lane idles where an ordinary item would leave 32-63 lanes
+// of the chunk empty. The groups' query values differ between the lanes, so each lane loads a
+// slab of 16 dims of its group's query (lane l holds dim l mod 16 of the slab; W is a multiple of
+// 16, so a row of 16 lanes belongs to one group) and the subtraction takes dim t of the slab from
+// lane t of its row by a DPP row broadcast, folded into the subtraction (v_sub_f32_dpp
+// row_newbcast). Every lane runs every instruction (the DPP reads need the whole row); only the
+// stores are predicated. Each value comes from the same operations in the same order as in
+// meso_summarize.
+template <uint32_t T>
+__device__ __forceinline__ float row_bcast(float v) {
+    // row_newbcast always reads a lane of the row, so no lane takes the "old" value: none is given
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + T, 0xF, 0xF, true));
+}
+
+// Vertical dims [lo, hi) of a 16-dim slab (wave-uniform bounds), in dim order.
+template <uint32_t T>
+__device__ __forceinline__ void tail_vslab(float (&acc)[kMesoQ], const float (&vq)[kMesoQ], const float (&x)[16],
+                                           uint32_t lo, uint32_t hi) {
+    if (T >= lo && T < hi) {
+#pragma unroll
+        for (uint32_t u = 0; u < kMesoQ; ++u) {
+            const float df = row_bcast<T>(vq[u]) - x[T];
+            acc[u] = __builtin_fmaf(df, df, acc[u]);
+        }
+    }
+    if constexpr (T < 15) tail_vslab<T + 1>(acc, vq, x, lo, hi);
+}
+
+// One 64-dim horizontal block in the 16-lane order: p[l] over dims l, l + 16, l + 32, l + 48;
+// vq[r] holds dims [16 r, 16 r + 16) of the block for this lane's row.
+template <uint32_t L>
+__device__ __forceinline__ void tail_hblock(float (&p)[16], const float (&vq)[4], const float (&xb)[kHDim]) {
+    float df = row_bcast<L>(vq[0]) - xb[L];
+    p[L] = __builtin_fmaf(df, df, 0.0f);
+    df = row_bcast<L>(vq[1]) - xb[16 + L];
+    p[L] = __builtin_fmaf(df, df, p[L]);
+    df = row_bcast<L>(vq[2]) - xb[32 + L];
+    p[L] = __builtin_fmaf(df, df, p[L]);
+    df = row_bcast<L>(vq[3]) - xb[48 + L];
+    p[L] = __builtin_fmaf(df, df, p[L]);
+    if constexpr (L < 15) tail_hblock<L + 1>(p, vq, xb);
+}
+
+// A packed tail item runs as kTailParts waves: part 0 the vertical chain and its checkpoints,
+// parts 1 .. kTailParts - 1 a third each of the horizontal blocks (independent of each other), so
+// the few packed items still fill the GPU.
+constexpr uint32_t kTailParts = 4;
+
+__device__ __forceinline__ void meso_tail_item(const MesoArgs& a, uint32_t item, uint32_t part) {
+    const uint32_t nc = a.L.nc;
+    uint32_t lo = 0, hi = nc;  // tstart[lo] <= item < tstart[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.tstart[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t m = lo;
+    const uint32_t n = a.L.n_used[m];  // > 0: the cluster has a packed tail item
+    const uint32_t tg = meso_tail_groups(n), W = kWave / tg;
+    const uint32_t i0 = (n - 1) / kWave * kWave, mt = n - i0;  // the last chunk: vectors [i0, n)
+    const uint32_t lane = threadIdx.x % kWave, col = lane % W;
+    const uint32_t l0 = a.lstart[m], l1 = a.lstart[m + 1];
+    const uint32_t e0 = l0 + ((item - a.tstart[m]) * tg + lane / W) * kMesoQ;  // this lane's group
+    const uint32_t ne = e0 < l1 ? min(kMesoQ, l1 - e0) : 0u;
+    const uint32_t i = i0 + min(col, mt - 1);
+    const bool valid = col < mt;
+    const uint32_t d = a.L.d, S = a.L.stride[m], v = a.L.v, nh = a.L.h / kHDim;
+    const uint32_t sn = (n + 3u) & ~3u;
+    const float* data = reinterpret_cast<const float*>(a.L.data + a.L.data_off[m]);
+
+    // per lane: row offset of each query of its group and its summary block (~0u = not stored)
+    uint32_t qo[kMesoQ], ob[kMesoQ];
+#pragma unroll
+    for (uint32_t u = 0; u < kMesoQ; ++u) {
+        const bool ok = u < ne;
+        const uint32_t e = a.list[ok ? e0 + u : l0];
+        const uint32_t q = e / a.stride;
+        qo[u] = q * d;
+        const unsigned long long r0 = a.qbase[q], r1 = a.qbase[q + 1], so = a.soff[e];
+        ob[u] = (ok && r0 + so + sn <= r1 && r1 <= a.summ_cap) ? static_cast<uint32_t>(r0 + so) : ~0u;
+    }
+    const uint32_t sl = lane & 15;  // this lane's dim within a slab
+
+    if (part == 0) {
+        // vertical: the fma chain over dims [0, end_t), stored at every checkpoint
+        const float* colp = data + i;
+        float acc[kMesoQ];
+#pragma unroll
+        for (uint32_t u = 0; u < kMesoQ; ++u) acc[u] = 0.0f;
+        uint32_t j = 0, end = 0;
+        for (uint32_t t = 0; t < a.nw; ++t) {
+            end = min(end + c_fetch[t], v);
+            while (j < end) {
+                const uint32_t jb = j & ~15u, top = min(end - jb, 16u);
+                float vq[kMesoQ];
+#pragma unroll
+                for (uint32_t u = 0; u < kMesoQ; ++u) vq[u] = a.qf[qo[u] + min(jb + sl, d - 1)];
+                float x[16];
+#pragma unroll
+                for (uint32_t s = 0; s < 16; ++s) x[s] = s < top ? colp[(size_t)(jb + s) * S] : 0.0f;
+                tail_vslab<0>(acc, vq, x, j - jb, top);
+                j = jb + top;
+            }
+            if (valid) {
+#pragma unroll
+                for (uint32_t u = 0; u < kMesoQ; ++u)
+                    if (ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)t * sn + i] = acc[u];
+            }
+        }
+        return;
+    }
+    // horizontal blocks of this part
+    const bool aligned = ((size_t)v * S) % 4 == 0;
+    const uint32_t b0 = (part - 1) * nh / (kTailParts - 1), b1 = part * nh / (kTailParts - 1);
+    for (uint32_t b = b0; b < b1; ++b) {
+        const float* blk = data + (size_t)(v + b * kHDim) * S + (size_t)i * kHDim;
+        float xb[kHDim];
+        if (aligned) {
+#pragma unroll
+            for (uint32_t r = 0; r < kHDim / 4; ++r) {
+                const float4 t4 = reinterpret_cast<const float4*>(blk)[r];
+                xb[4 * r] = t4.x;
+                xb[4 * r + 1] = t4.y;
+                xb[4 * r + 2] = t4.z;
+                xb[4 * r + 3] = t4.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t l = 0; l < kHDim; ++l) xb[l] = blk[l];
+        }
+        const uint32_t hb = v + b * kHDim + sl;
+#pragma unroll
+        for (uint32_t u = 0; u < kMesoQ; ++u) {
+            float vq[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) vq[r] = a.qf[qo[u] + hb + 16 * r];
+            float p[16];
+            tail_hblock<0>(p, vq, xb);
+            float hv = tree16_regs(p);
+            // keeps the block's arithmetic ahead of the predicated store (sunk into it, the
+            // subtractions could no longer take the row broadcasts, which must run on every lane)
+            asm volatile("" : "+v"(hv));
+            if (valid && ob[u] != ~0u) a.summ[(size_t)ob[u] * a.ns + (size_t)(a.nw + b) * sn + i] = hv;
+        }
+    }
+}
+
+// One part of a packed tail item per wave (the parts of an item adjacent), dealt to the XCDs as in
+// k_meso_summary.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_meso_tails(MesoArgs a) {
+    const uint32_t total = a.tstart[a.L.nc] * kTailParts;
+    const uint32_t per = (total + 7) / 8;
+    const uint32_t local = blockIdx.x / 8 * kMesoWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t item = blockIdx.x % 8 * per + local;
+    if (local < per && item < total) meso_tail_item(a, item / kTailParts, item % kTailParts);
 }
 
 }  // namespace pdxg
