@@ -688,7 +688,21 @@ def bench_dense(args, cfg, torch, _capi, builder, dev):
         live[:n] = True
         per = [n_up // args.steps + (1 if i < n_up % args.steps else 0) for i in range(args.steps)]
         urng = np.random.default_rng(args.seed + 3)
-        nxt = 0
+        # the update schedule (rows appended and ids deleted at each step) is synthetic input: drawn
+        # before the timed loop, so the timed updates are the library's append and delete calls
+        sched, nxt = [], 0
+        for i in range(args.steps):
+            a = per[i]
+            if not a:
+                sched.append(None)
+                continue
+            new_ids = np.arange(n + nxt, n + nxt + a, dtype=np.uint32)
+            live[new_ids] = True
+            cand = np.nonzero(live[:n])[0]
+            gone = urng.choice(cand, a, replace=False).astype(np.uint32)
+            live[gone] = False
+            sched.append((np.ascontiguousarray(x_new[nxt:nxt + a]), new_ids, gone))
+            nxt += a
         t_search = t_upd = 0.0
         stage = np.zeros(4)
         gemm_rows = 0
@@ -700,16 +714,10 @@ def bench_dense(args, cfg, torch, _capi, builder, dev):
             stage += dn.stage_times()
             gemm_rows += dn.info()["gemm_rows"]
             t = time.perf_counter()
-            a = per[i]
-            if a:
-                new_ids = np.arange(n + nxt, n + nxt + a, dtype=np.uint32)
-                dn.append(x_new[nxt:nxt + a], new_ids)
-                live[new_ids] = True
-                nxt += a
-                cand = np.nonzero(live[:n])[0]
-                gone = urng.choice(cand, a, replace=False).astype(np.uint32)
+            if sched[i] is not None:
+                rows_i, new_ids, gone = sched[i]
+                dn.append(rows_i, new_ids)
                 dn.delete(gone)
-                live[gone] = False
             t_upd += time.perf_counter() - t
         stage /= args.steps
         gemm_rows /= args.steps
