@@ -70,14 +70,27 @@ def test_search_matches_oracle(gpu_lib, case):
             _compare(o_res, g_res, o.is_u8)
 
 
-@pytest.mark.parametrize("case", CASES[:5], ids=_ids)
+def sq8_quantize(rot, base, scale):
+    """The SQ8 form of a rotated query (searcher.hpp:638-643 applies QuantizeEmbedding,
+    scalar.hpp:76-94): std::round((x - base) * scale) with the float operations of the reference,
+    rounding half away from zero, clamped to [0, 255]."""
+    v = ((rot - np.float32(base)) * np.float32(scale)).astype(np.float64)
+    r = np.sign(v) * np.floor(np.abs(v) + 0.5)
+    return np.clip(r, 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("case", CASES[:5] + [CASES[6], CASES[7]], ids=_ids)
 def test_preprocess_matches_oracle(gpu_lib, case):
     kind, n, d, nlist, norm = case
     path, x, q = index_file(kind, n, d, nlist, normalize=norm)
     o = OracleIndex(path)
-    g = gpu_lib.GpuIndex(pf.load(path))
+    idx = pf.load(path)
+    g = gpu_lib.GpuIndex(idx)
     rot, qu = g.preprocess(q)
-    np.testing.assert_array_equal(rot.view(np.uint32), o.preprocess(q).view(np.uint32))
+    orot = o.preprocess(q)
+    np.testing.assert_array_equal(rot.view(np.uint32), orot.view(np.uint32))
+    if qu is not None:  # the quantised query the U8 scan uses, from the oracle's rotated query
+        np.testing.assert_array_equal(qu, sq8_quantize(orot, idx.l1.base, idx.l1.scale))
     # a batch large enough for the tiled rotation (64 queries x 64 outputs per tile, used from 1024
     # queries when there are two tiles per CU of the MI355X's 256), with a ragged last tile; its
     # first rows also go through the 16-query kernel (1089 queries make too few tiles at this d)
@@ -86,7 +99,10 @@ def test_preprocess_matches_oracle(gpu_lib, case):
     nb = 64 * -(-512 // ((d + 63) // 64)) + 65
     qb = np.ascontiguousarray(x[rng.integers(0, n, nb)] + np.float32(0.01) * rng.standard_normal((nb, d), dtype=np.float32))
     rot_b, qu_b = g.preprocess(qb)
-    np.testing.assert_array_equal(rot_b.view(np.uint32), o.preprocess(qb).view(np.uint32))
+    orot_b = o.preprocess(qb)
+    np.testing.assert_array_equal(rot_b.view(np.uint32), orot_b.view(np.uint32))
+    if qu_b is not None:  # the tiled rotation's fused quantisation
+        np.testing.assert_array_equal(qu_b, sq8_quantize(orot_b, idx.l1.base, idx.l1.scale))
     np.testing.assert_array_equal(rot_b[:1089].view(np.uint32), g.preprocess(qb[:1089])[0].view(np.uint32))
     np.testing.assert_array_equal(rot_b[:len(q)].view(np.uint32), g.preprocess(qb[:len(q)])[0].view(np.uint32))
     # one and a few queries (the wide small-batch rotation)
